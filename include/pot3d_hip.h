@@ -1,0 +1,180 @@
+/*
+ * pot3d_hip.h -- C-ABI of libpot3d_hip.so, the gfx950 (MI355X) solver for
+ * POT3D's preconditioned conjugate-gradient solve of the finite-difference
+ * spherical Laplace equation.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The legacy external-LU-solver ABI, symbol for symbol the one that
+ *     POT3D binds under -DHIPSPARSE (src/pot3d.F90:581-609, implemented by
+ *     src/lusol_hipsparse.c).  An unmodified pot3d.F90 built with
+ *     -DHIPSPARSE links against this library instead of the hipSPARSE shim
+ *     and runs ifprec=2 through the D-ILU factorisation and trisolves here.
+ *
+ *  2. A whole-solve ABI that replaces the do-concurrent solver loops of
+ *     potfld/solve/cgsolve (src/pot3d.F90:3866-4485, 5102-5844): the
+ *     operator, boundary values, phi seam, dots, preconditioners and the
+ *     CG iteration all run on the GPU.  The driver keeps the mesh, Br input,
+ *     I/O and diagnostics and passes host arrays.
+ *
+ * Conventions (group 2): every function returns 0 (P3D_OK) or an error
+ * code; p3d_last_error() describes the last failure on the calling thread.
+ * Arrays are host pointers unless the name ends in _dev.  1-D mesh arrays
+ * are the reference's 1-based Fortran arrays passed as plain pointers
+ * (element (1) first).  Packed solver vectors have N=(nr-2)(nt-2)(np-2)
+ * entries, r fastest (the layout of x_cg/rhs_cg, src/pot3d.F90:3917-3924);
+ * padded arrays are (nr,nt,np) in Fortran order; br0/br1 are (nt,np).
+ * All calls are synchronous at return.  A solver handle must not be used
+ * by two threads at once.
+ */
+#ifndef POT3D_HIP_H
+#define POT3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* 1. Legacy external LU solver ABI (drop-in for src/lusol_hipsparse.c) */
+
+/* Replaces load_external_lu_solver (interface src/pot3d.F90:586-592,
+ * implementation src/lusol_hipsparse.c:70-300).  CSR of the reference's
+ * ILU matrix (diacsr, src/pot3d.F90:4870-5018): 1-based, FP32 values,
+ * int32 indices, N rows, M = nnz+1 (the reference's allocated length).
+ * Pointers may be device addresses (the -DCUSPARSE calling convention)
+ * or host addresses (the -DHIPSPARSE convention; the arrays are read from
+ * host memory).  The 7-point + phi-wrap structure is recognised from IA/JA
+ * and factorised as D-ILU with FP64 pivots; the LU factors are written
+ * back over csr_lu in FP32, as the reference's in-place csrilu02 does.
+ * On an unsupported pattern or any error: message on stderr, exit(1)
+ * (the reference shim's behaviour). */
+void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M);
+
+/* Replaces external_lu_solver (src/pot3d.F90:594-599,
+ * src/lusol_hipsparse.c:334-363): x <- (LU)^-1 x in place, FP32 inside,
+ * x a device (or host) address of N doubles; synchronous on return. */
+void external_lu_solver(double* x);
+
+/* Replaces unload_external_lu_solver (src/pot3d.F90:601-603,
+ * src/lusol_hipsparse.c:369-384): frees all state of the legacy solver. */
+void unload_external_lu_solver(void);
+
+/* ------------------------------------------------------------------ */
+/* 2. Whole-solve ABI                                                   */
+
+typedef struct p3d_solver p3d_solver;
+
+enum {
+  P3D_OK = 0,
+  P3D_EINVAL = 1,     /* bad argument or shape */
+  P3D_ENOMEM = 2,     /* device allocation failed */
+  P3D_EHIP = 3,       /* HIP runtime error */
+  P3D_ENOCONV = 4,    /* ncgmax reached (the reference's ierr=1) */
+  P3D_ENONFINITE = 5, /* NaN/Inf in the residual (e.g. non-finite br0) */
+  P3D_ESTATE = 6      /* call out of order (e.g. solve before set_br0) */
+};
+
+/* Preconditioner choice: the reference's ifprec (src/pot3d.F90:428). */
+enum { P3D_PREC_NONE = 0, P3D_PREC_DIAG = 1, P3D_PREC_ILU0 = 2 };
+
+/* Options for p3d_set_option. */
+enum {
+  P3D_OPT_STOP = 1,       /* 0: epsn = sqrt(r.M^-1r / b.M^-1b) < epscg (the
+                             reference's ernorm, default); 1: ||r||2/||b||2 */
+  P3D_OPT_MIXED = 2,      /* 1: FP32 inner PCG inside an FP64 defect-
+                             correction loop to ||b-Ax||2/||b||2 <= epscg */
+  P3D_OPT_INNER_TOL = 3,  /* inner relative tolerance (default 1e-5) */
+  P3D_OPT_CHUNK = 4,      /* CG iterations per captured graph (default auto) */
+  P3D_OPT_GRAPH = 5,      /* 1: replay iterations as a hipGraph (default) */
+  P3D_OPT_SWEEP = 6,      /* ILU sweep: 0 dataflow (default), 1 hyperplane
+                             launches */
+  P3D_OPT_VERBOSE = 7     /* 1: print the reference's residual lines */
+};
+
+/* Creates a solver for one rank's grid (nr,nt,np include the boundary /
+ * ghost points, as in the reference's local_dims).  Mesh arrays, 1-based
+ * Fortran arrays of src/pot3d.F90:2730-2861:
+ *   r, dr      (nr-1)   main r mesh and its spacing
+ *   rh, drh    (nr)     half r mesh and its spacing
+ *   st, dt     (nt-1)   sin(t) of the main theta mesh, spacing
+ *   sth, dth   (nt)     sin(th) of the half theta mesh, spacing
+ *   dp, dph    (np)     phi spacings of the main and half meshes
+ * pm_r1: +1 for option 'potential', -1 for 'open'/'ss' (:1250-1285).
+ * ifprec: P3D_PREC_*.  The operator is matrix-free: no coefficient arrays
+ * are stored (the reference stores 7 FP64 planes, :4506). */
+int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const double* dr,
+               const double* rh, const double* drh, const double* st, const double* dt,
+               const double* sth, const double* dth, const double* dp, const double* dph,
+               double pm_r1, int ifprec);
+int p3d_destroy(p3d_solver* s);
+const char* p3d_last_error(void);
+int p3d_set_option(p3d_solver* s, int key, double value);
+
+/* Boundary data: br0(nt,np) at r0 (module fields, src/pot3d.F90:399);
+ * optional br1(nt,np) selects the validation-run Neumann value at r1
+ * (:5420-5423).  Pass br1=NULL to clear it. */
+int p3d_set_br0(p3d_solver* s, const double* br0);
+int p3d_set_br1(p3d_solver* s, const double* br1);
+
+/* rhs(N) of the boundary trick (src/pot3d.F90:3941-3950). */
+int p3d_rhs(p3d_solver* s, double* rhs);
+
+/* The reference's solve(x,rhs,N,ierr) (src/pot3d.F90:4169-4229 ->
+ * cgsolve :4231-4369): x(N) initial guess in, solution out; rhs(N) in,
+ * residual b-Ax out.  rhs may be NULL: the boundary-trick rhs of the
+ * current br0/br1 is used (and the residual is not returned).  ncghist>0
+ * prints the reference's residual lines every ncghist iterations when
+ * P3D_OPT_VERBOSE is set; the history is also kept for p3d_history.
+ * Returns P3D_OK, P3D_ENOCONV (ierr=1), P3D_ENONFINITE, ...; *ncg and
+ * *epsn as the reference's cgcom variables. */
+int p3d_solve(p3d_solver* s, double* x, double* rhs, double epscg, int ncgmax, int ncghist,
+              int* ncg, double* epsn);
+/* Same with device pointers (N doubles each; rhs_dev NULL as above). */
+int p3d_solve_dev(p3d_solver* s, double* x_dev, double* rhs_dev, double epscg, int ncgmax,
+                  int* ncg, double* epsn);
+
+/* After a solve: phi(nr,nt,np) = unpack(x) + boundary values (vmask=1) +
+ * phi seam (src/pot3d.F90:3966-3969).  x NULL uses the last solution. */
+int p3d_fill_phi(p3d_solver* s, const double* x, double* phi);
+
+/* getb + magnetic_energy (src/pot3d.F90:6187-6321) of phi (host, NULL =
+ * the phi of the last p3d_fill_phi): w[0..2] = energy in Br^2, Bt^2, Bp^2.
+ * br/bt/bp may be NULL; else receive (nr-1,nt,np), (nr,nt-1,np),
+ * (nr,nt,np-1). */
+int p3d_energy(p3d_solver* s, const double* phi, double* w, double* br, double* bt, double* bp);
+
+/* Single operations (host vectors of N), for parity checks:
+ * y = A x (ax, :5102-5150); x <- M^-1 x (prec_inv, :5152-5225). */
+int p3d_apply_A(p3d_solver* s, const double* x, double* y);
+int p3d_apply_prec(p3d_solver* s, double* x);
+/* ILU pivot inverses real(1/d, REAL32) of the D-ILU factorisation (N). */
+int p3d_ilu_dinv(p3d_solver* s, float* dinv);
+
+/* Statistics of the last solve. */
+typedef struct {
+  int ncg;              /* CG iterations (sum of inner iterations if mixed) */
+  int n_outer;          /* defect-correction sweeps (mixed), else 0 */
+  int status;           /* 1 converged, 2 ncgmax, 3 non-finite, 4 zero rhs */
+  int pad_;
+  double epsn;          /* final epsn (reference definition) */
+  double relres;        /* final ||r||2/||b||2 of the recurrence residual */
+  double true_relres;   /* ||b - A x||2/||b||2 recomputed in FP64 */
+  double t_solve;       /* seconds, p3d_solve entry to return */
+  double t_setup;       /* seconds spent in preconditioner set-up */
+} p3d_stats;
+int p3d_get_stats(p3d_solver* s, p3d_stats* st);
+/* epsn per iteration 0..n-1 of the last (inner) solve; returns count. */
+int p3d_history(p3d_solver* s, double* hist, int n);
+
+/* Kernel timing with HIP events on the solver's stream: `which` =
+ * 0 FP64 operator, 1 FP32 operator, 2 ILU apply (FP64 in/out),
+ * 3 one FP64 PCG iteration, 4 FP64 CG vector update, 5 FP64 p update.
+ * Average milliseconds per launch over reps launches. */
+int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POT3D_HIP_H */

@@ -1,0 +1,684 @@
+/*
+ * p3d_oracle.c -- CPU restatement of POT3D's preconditioned conjugate-
+ * gradient solve of the finite-difference spherical Laplace equation.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it;
+ * the product library (libpot3d_hip.so) never links, loads or calls it.
+ *
+ * Every routine names the reference routine it restates (file:line in
+ * predsci/POT3D).  Layout follows the reference exactly:
+ *   padded 3-D arrays s(nr,nt,np), Fortran order, 1-based:
+ *       s(i,j,k) = s[(i-1) + nr*((j-1) + nt*(k-1))]
+ *   packed interior vectors x(N), N=(nr-2)(nt-2)(np-2), r fastest
+ *   (src/pot3d.F90:3907-3917, 5323):
+ *       x(i,j,k) = x[(i-2) + a*((j-2) + b*(k-2))], a=nr-2, b=nt-2
+ *   2-D boundary arrays br0(nt,np): br0[(j-1) + nt*(k-1)]
+ *   1-D mesh arrays are passed 0-based: R(i) = r[i-1].
+ * Built with -ffp-contract=off so products and sums round in the order
+ * they are written, which is the order of the reference's Fortran source.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   coefficients, ax, rhs  -- explicit scipy sparse assembly, symmetry,
+ *                             row sums, closed-form RHS, tiny brute force
+ *   getM/diacsr/ilu0/lusol -- dense textbook ILU(0) and dense triangular
+ *                             solves on tiny grids
+ *   dilu_*                 -- equality with ilu0 (to FP32 rounding) and
+ *                             with dense ILU(0) in FP64
+ *   cgsolve                -- dense direct solve on tiny grids; the three
+ *                             testsuite decks' energies and iteration counts
+ *   getb/energy            -- closed-form fields on uniform meshes; the
+ *                             testsuite energies
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  int nr, nt, np;
+  const double *r, *dr;    /* nr-1 entries: main r mesh r(1:nrm1), dr(1:nrm1) */
+  const double *rh, *drh;  /* nr entries: half r mesh */
+  const double *st, *dt;   /* nt-1 entries: sin(t(1:ntm1)), dt(1:ntm1) */
+  const double *sth, *dth; /* nt entries: sin(th(1:nt)), dth(1:nt) */
+  const double *dp, *dph;  /* np entries each (periodic phi) */
+  double pm_r1;            /* +1 'potential', -1 'open'/'ss' */
+} orc_grid;
+
+#define PI_F90 3.1415926535897932
+/* pl_i = one/pl, pl = two*pi - 0 (src/pot3d.F90:192-197) */
+static double pl_i(void) { return 1.0 / (2.0 * PI_F90 - 0.0); }
+
+#define IDX3(i, j, k) ((size_t)((i)-1) + (size_t)nr * ((size_t)((j)-1) + (size_t)nt * (size_t)((k)-1)))
+#define PIDX(i, j, k) ((size_t)((i)-2) + (size_t)A * ((size_t)((j)-2) + (size_t)B * (size_t)((k)-2)))
+
+/* ---------------------------------------------------------------------- */
+/* Deterministic FP64 dot product: fixed 4096-element chunks summed in
+ * order, so the result does not depend on the thread count.            */
+static double det_dot(const double *x, const double *y, size_t n) {
+  const size_t CH = 4096;
+  size_t nch = (n + CH - 1) / CH;
+  double *part = (double *)malloc(sizeof(double) * (nch ? nch : 1));
+  long c;
+#pragma omp parallel for schedule(static)
+  for (c = 0; c < (long)nch; c++) {
+    size_t lo = (size_t)c * CH, hi = lo + CH < n ? lo + CH : n;
+    double s = 0.0;
+    for (size_t i = lo; i < hi; i++) s = s + x[i] * y[i];
+    part[c] = s;
+  }
+  double s = 0.0;
+  for (size_t c2 = 0; c2 < nch; c2++) s = s + part[c2];
+  free(part);
+  return s;
+}
+
+double orc_dot(const double *x, const double *y, int64_t n) { return det_dot(x, y, (size_t)n); }
+
+/* ---------------------------------------------------------------------- */
+/* load_matrix_pot3d_solve: src/pot3d.F90:4565-4613.
+ * a[m*N + n], m=0..6 holds a(i,j,k,m+1).                                */
+void orc_coefs(const orc_grid *g, double *a) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2, C = np - 2;
+  const size_t N = (size_t)A * B * C;
+  (void)C;
+  long k;
+#pragma omp parallel for schedule(static)
+  for (k = 2; k <= np - 1; k++) {
+    for (int j = 2; j <= nt - 1; j++) {
+      for (int i = 2; i <= nr - 1; i++) {
+        const size_t n = PIDX(i, j, k);
+        const double drh = g->drh[i - 1], dth = g->dth[j - 1], sth = g->sth[j - 1];
+        const double sth_i = 1.0 / sth;
+        const double dph = g->dph[k - 1];
+        const double dpi_km1 = 1.0 / g->dp[k - 2], dpi_k = 1.0 / g->dp[k - 1];
+        const double st_jm1 = g->st[j - 2], st_j = g->st[j - 1];
+        const double dti_jm1 = 1.0 / g->dt[j - 2], dti_j = 1.0 / g->dt[j - 1];
+        const double r2_im1 = g->r[i - 2] * g->r[i - 2], r2_i = g->r[i - 1] * g->r[i - 1];
+        const double dri_im1 = 1.0 / g->dr[i - 2], dri_i = 1.0 / g->dr[i - 1];
+        const double a1 = -(drh * dth * sth_i * dpi_km1);
+        const double a2 = -(drh * dph * st_jm1 * dti_jm1);
+        const double a3 = -(sth * dth * dph * r2_im1 * dri_im1);
+        const double a5 = -(sth * dth * dph * r2_i * dri_i);
+        const double a6 = -(drh * dph * st_j * dti_j);
+        const double a7 = -(drh * dth * sth_i * dpi_k);
+        const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
+        a[0 * N + n] = a1;
+        a[1 * N + n] = a2;
+        a[2 * N + n] = a3;
+        a[3 * N + n] = a4;
+        a[4 * N + n] = a5;
+        a[5 * N + n] = a6;
+        a[6 * N + n] = a7;
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* set_boundary_points: src/pot3d.F90:5376-5487 (one rank: rb0, rb1, tb0,
+ * tb1 all true; sum_over_phi is then the identity).  br1 may be NULL
+ * (the validation-run Neumann branch at r1, :5420-5423, uses it).       */
+void orc_set_boundary_points(const orc_grid *g, const double *br0, const double *br1,
+                             double vmask, double *x) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const double dr1 = g->dr[0], drn = g->dr[nr - 2];
+  const double pli = pl_i();
+  for (int k = 2; k <= np - 1; k++)
+    for (int j = 2; j <= nt - 1; j++) {
+      const double b0 = br0 ? br0[(j - 1) + (size_t)nt * (k - 1)] : 0.0;
+      x[IDX3(1, j, k)] = x[IDX3(2, j, k)] - vmask * b0 * dr1;
+    }
+  for (int k = 2; k <= np - 1; k++)
+    for (int j = 2; j <= nt - 1; j++) {
+      if (br1)
+        x[IDX3(nr, j, k)] = x[IDX3(nr - 1, j, k)] + vmask * br1[(j - 1) + (size_t)nt * (k - 1)] * drn;
+      else
+        x[IDX3(nr, j, k)] = g->pm_r1 * x[IDX3(nr - 1, j, k)];
+    }
+  double *sum0 = (double *)malloc(sizeof(double) * nr);
+  double *sum1 = (double *)malloc(sizeof(double) * nr);
+  for (int i = 1; i <= nr; i++) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 2; k <= np - 1; k++) {
+      s0 = s0 + x[IDX3(i, 2, k)] * g->dph[k - 1] * pli;
+      s1 = s1 + x[IDX3(i, nt - 1, k)] * g->dph[k - 1] * pli;
+    }
+    sum0[i - 1] = s0;
+    sum1[i - 1] = s1;
+  }
+  for (int k = 2; k <= np - 1; k++)
+    for (int i = 1; i <= nr; i++) {
+      x[IDX3(i, 1, k)] = 2.0 * sum0[i - 1] - x[IDX3(i, 2, k)];
+      x[IDX3(i, nt, k)] = 2.0 * sum1[i - 1] - x[IDX3(i, nt - 1, k)];
+    }
+  free(sum0);
+  free(sum1);
+}
+
+/* seam_hhh, periodic phi at one rank: src/pot3d.F90:5738-5758. */
+void orc_seam(const orc_grid *g, double *x) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const size_t plane = (size_t)nr * nt;
+  memcpy(x + 0 * plane, x + (size_t)(np - 2) * plane, plane * sizeof(double));
+  memcpy(x + (size_t)(np - 1) * plane, x + 1 * plane, plane * sizeof(double));
+}
+
+/* unpack_scalar: src/pot3d.F90:5304-5335. */
+void orc_unpack(const orc_grid *g, const double *xp, double *s) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2;
+  long k;
+#pragma omp parallel for schedule(static)
+  for (k = 2; k <= np - 1; k++)
+    for (int j = 2; j <= nt - 1; j++)
+      for (int i = 2; i <= nr - 1; i++) s[IDX3(i, j, k)] = xp[PIDX(i, j, k)];
+}
+
+/* delsq: src/pot3d.F90:5337-5374. */
+void orc_delsq(const orc_grid *g, const double *a, const double *x, double *y) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2, C = np - 2;
+  const size_t N = (size_t)A * B * C;
+  long k;
+#pragma omp parallel for schedule(static)
+  for (k = 2; k <= np - 1; k++)
+    for (int j = 2; j <= nt - 1; j++)
+      for (int i = 2; i <= nr - 1; i++) {
+        const size_t n = PIDX(i, j, k);
+        y[n] = a[0 * N + n] * x[IDX3(i, j, k - 1)] + a[1 * N + n] * x[IDX3(i, j - 1, k)] +
+               a[2 * N + n] * x[IDX3(i - 1, j, k)] + a[3 * N + n] * x[IDX3(i, j, k)] +
+               a[4 * N + n] * x[IDX3(i + 1, j, k)] + a[5 * N + n] * x[IDX3(i, j + 1, k)] +
+               a[6 * N + n] * x[IDX3(i, j, k + 1)];
+      }
+}
+
+/* ax: src/pot3d.F90:5102-5150.  xa is the padded scratch x_ax(nr,nt,np). */
+void orc_ax(const orc_grid *g, const double *a, const double *x, double *y, double *xa) {
+  orc_unpack(g, x, xa);
+  orc_set_boundary_points(g, NULL, NULL, 0.0, xa);
+  orc_seam(g, xa);
+  orc_delsq(g, a, xa, y);
+}
+
+/* RHS "boundary trick": src/pot3d.F90:3941-3950, starting from x_ax = 0. */
+void orc_rhs(const orc_grid *g, const double *a, const double *br0, const double *br1,
+             double *rhs, double *xa) {
+  const size_t tot = (size_t)g->nr * g->nt * g->np;
+  const size_t N = (size_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  memset(xa, 0, tot * sizeof(double));
+  orc_set_boundary_points(g, br0, br1, 1.0, xa);
+  orc_seam(g, xa);
+  orc_delsq(g, a, xa, rhs);
+  for (size_t n = 0; n < N; n++) rhs[n] = -rhs[n];
+}
+
+/* Post-solve phi: unpack, inhomogeneous BCs, seam (src/pot3d.F90:3966-3969). */
+void orc_fill_phi(const orc_grid *g, const double *x, const double *br0, const double *br1,
+                  double *phi) {
+  orc_unpack(g, x, phi);
+  orc_set_boundary_points(g, br0, br1, 1.0, phi);
+  orc_seam(g, phi);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Diagonal preconditioner: a_i = real(one/a4, REAL32) (src/pot3d.F90:
+ * 4649-4661); apply x = a_i*x (:5189-5191).                             */
+void orc_diag_prec(const orc_grid *g, const double *a, float *a_i) {
+  const size_t N = (size_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  for (size_t n = 0; n < N; n++) a_i[n] = (float)(1.0 / a[3 * N + n]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* getM: src/pot3d.F90:5020-5100 (nproc_p = 1).  IA is 1-based, N+1 long;
+ * returns M = IA(N+1) = nnz+1, the reference's allocated length.        */
+int64_t orc_getM(const orc_grid *g, int *IA) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  size_t i = 0;
+  IA[0] = 1;
+  for (int mk = 2; mk <= np - 1; mk++)
+    for (int mj = 2; mj <= nt - 1; mj++)
+      for (int mi = 2; mi <= nr - 1; mi++) {
+        int ok[7] = {1, 1, 1, 1, 1, 1, 1};
+        if (mi == 2) ok[2] = 0;
+        if (mi == nr - 1) ok[4] = 0;
+        if (mj == 2) ok[1] = 0;
+        if (mj == nt - 1) ok[5] = 0;
+        int cnt = 0;
+        for (int jj = 0; jj < 7; jj++) cnt += ok[jj];
+        IA[i + 1] = IA[i] + cnt;
+        i++;
+      }
+  return IA[i];
+}
+
+/* diacsr: src/pot3d.F90:4870-5018 (nproc_p = 1: periodic wrap kept).
+ * Acsr FP32, JA and Adptr 1-based.                                       */
+void orc_diacsr(const orc_grid *g, const double *a, const int *IA, float *Acsr, int *JA,
+                int *Adptr) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2, C = np - 2;
+  const int64_t N = (int64_t)A * B * C;
+  const int64_t ioff[7] = {-(int64_t)A * B, -A, -1, 0, 1, A, (int64_t)A * B};
+  for (int mk = 2; mk <= np - 1; mk++)
+    for (int mj = 2; mj <= nt - 1; mj++)
+      for (int mi = 2; mi <= nr - 1; mi++) {
+        const int64_t i = (int64_t)(mk - 2) * B * A + (int64_t)(mj - 2) * A + (mi - 1);
+        int64_t ko = IA[i - 1];
+        int ok[7] = {1, 1, 1, 1, 1, 1, 1};
+        if (mi == 2) ok[2] = 0;
+        if (mi == nr - 1) ok[4] = 0;
+        if (mj == 2) ok[1] = 0;
+        if (mj == nt - 1) ok[5] = 0;
+        for (int jj = 0; jj < 7; jj++)
+          if (ok[jj]) {
+            int64_t j = i + ioff[jj];
+            if (j > N) {
+              j -= N;
+              Acsr[ko - 1] = (float)a[jj * N + (i - 1)];
+              JA[ko - 1] = (int)j;
+              ko++;
+            }
+          }
+        for (int jj = 0; jj < 7; jj++)
+          if (ok[jj]) {
+            int64_t j = i + ioff[jj];
+            if (j >= 1 && j <= N) {
+              if (jj == 3) Adptr[i - 1] = (int)ko;
+              Acsr[ko - 1] = (float)a[jj * N + (i - 1)];
+              JA[ko - 1] = (int)j;
+              ko++;
+            }
+          }
+        for (int jj = 0; jj < 7; jj++)
+          if (ok[jj]) {
+            int64_t j = i + ioff[jj];
+            if (j < 1) {
+              j += N;
+              Acsr[ko - 1] = (float)a[jj * N + (i - 1)];
+              JA[ko - 1] = (int)j;
+              ko++;
+            }
+          }
+        (void)C;
+      }
+}
+
+/* ilu0 (Saad IKJ, in place, FP32): src/pot3d.F90:4717-4813.
+ * Returns icode (0, or the 1-based row of a zero pivot).                 */
+int orc_ilu0(int64_t N, float *Av, const int *JA, const int *IA, const int *A_da) {
+  int64_t *iw = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+  int icode = 0;
+  for (int64_t i = 2; i <= N; i++) {
+    const int64_t IA_i = IA[i - 1], IA_ip1m1 = IA[i] - 1;
+    for (int64_t ij = IA_i; ij <= IA_ip1m1; ij++) iw[JA[ij - 1]] = ij;
+    int64_t ik;
+    for (ik = IA_i; ik <= A_da[i - 1] - 1; ik++) {
+      const int64_t k = JA[ik - 1];
+      const float Aik = Av[ik - 1] / Av[A_da[k - 1] - 1];
+      Av[ik - 1] = Aik;
+      for (int64_t kj = A_da[k - 1] + 1; kj <= IA[k] - 1; kj++) {
+        const int64_t ij = iw[JA[kj - 1]];
+        if (ij != 0) Av[ij - 1] = Av[ij - 1] - Aik * Av[kj - 1];
+      }
+    }
+    /* after the loop ik == A_da(i): the diagonal (Fortran DO-variable rule) */
+    if (Av[ik - 1] == 0.0f) {
+      icode = (int)i;
+      break;
+    }
+    for (int64_t ij = IA_i; ij <= IA_ip1m1; ij++) iw[JA[ij - 1]] = 0;
+  }
+  free(iw);
+  return icode;
+}
+
+/* lu2luopt: src/pot3d.F90:4815-4868. */
+void orc_lu2luopt(int64_t N, float *LU, const float *Av, const int *IA, const int *JA, int *LUJA,
+                  const int *A_da, int *N1, int *N2) {
+  int64_t ii = 0;
+  for (int64_t i = 1; i <= N; i++) {
+    for (int64_t k = IA[i - 1]; k <= A_da[i - 1] - 1; k++) {
+      ii++;
+      LU[ii - 1] = Av[k - 1];
+      LUJA[ii - 1] = JA[k - 1];
+    }
+    N1[i - 1] = A_da[i - 1] - 1 - IA[i - 1];
+    N2[i - 1] = IA[i] - 2 - A_da[i - 1];
+  }
+  for (int64_t i = N; i >= 1; i--)
+    for (int64_t k = A_da[i - 1] + 1; k <= IA[i] - 1; k++) {
+      ii++;
+      LU[ii - 1] = Av[k - 1];
+      LUJA[ii - 1] = JA[k - 1];
+    }
+}
+
+/* lusol (forward unit-L, backward U with inverse diagonal), FP32:
+ * src/pot3d.F90:5227-5302.                                               */
+void orc_lusol(int64_t N, float *x, const float *LU, const int *LUJA, const int *N1,
+               const int *N2, const float *LUd_i) {
+  int64_t k1, k2 = 0;
+  for (int64_t i = 1; i <= N; i++) {
+    k1 = k2 + 1;
+    k2 = k1 + N1[i - 1];
+    for (int64_t k = k1; k <= k2; k++) x[i - 1] = x[i - 1] - LU[k - 1] * x[LUJA[k - 1] - 1];
+  }
+  for (int64_t i = N; i >= 1; i--) {
+    k1 = k2 + 1;
+    k2 = k1 + N2[i - 1];
+    for (int64_t k = k1; k <= k2; k++) x[i - 1] = x[i - 1] - LU[k - 1] * x[LUJA[k - 1] - 1];
+    x[i - 1] = x[i - 1] * LUd_i[i - 1];
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* D-ILU: on the reference's ILU pattern (7-point stencil, r/theta
+ * boundary couplings dropped, phi wrap kept; np >= 6) ILU(0) fill lands
+ * only on the diagonal, so L and U keep A's off-diagonals and only the
+ * pivots change:  d_p = a4_p - sum_{q in lower(p)} (a_pq/d_q)*a_qp,
+ * summed in the column order ilu0 uses (wrap, k-1, j-1, i-1).
+ * FP64 here; the product path stores dinv = real(1/d, REAL32).          */
+void orc_dilu_pivots(const orc_grid *g, const double *a, double *d) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2, C = np - 2;
+  const size_t N = (size_t)A * B * C;
+  const size_t sj = (size_t)A, sk = (size_t)A * B;
+  for (int k = 0; k < C; k++)
+    for (int j = 0; j < B; j++)
+      for (int i = 0; i < A; i++) {
+        const size_t p = i + sj * j + sk * k;
+        double dp_ = a[3 * N + p];
+        if (k == C - 1) { /* wrap: q=(i,j,0); a_pq = a7(p), a_qp = a1(q) */
+          const size_t q = i + sj * j;
+          dp_ = dp_ - a[6 * N + p] / d[q] * a[0 * N + q];
+        }
+        if (k > 0) { /* q=(i,j,k-1): a_pq = a1(p), a_qp = a7(q) */
+          const size_t q = p - sk;
+          dp_ = dp_ - a[0 * N + p] / d[q] * a[6 * N + q];
+        }
+        if (j > 0) { /* q=(i,j-1,k): a2(p), a6(q) */
+          const size_t q = p - sj;
+          dp_ = dp_ - a[1 * N + p] / d[q] * a[5 * N + q];
+        }
+        if (i > 0) { /* q=(i-1,j,k): a3(p), a5(q) */
+          const size_t q = p - 1;
+          dp_ = dp_ - a[2 * N + p] / d[q] * a[4 * N + q];
+        }
+        d[p] = dp_;
+      }
+}
+
+/* D-ILU apply in FP32, the arithmetic of the product's trisolve kernels:
+ *   forward  y_p = r_p - sum_{q<p} a_pq * (dinv_q * y_q)   (wrap,k-1,j-1,i-1)
+ *   backward z_p = (y_p - sum_{q>p} a_pq * z_q) * dinv_p    (i+1,j+1,k+1,wrap)
+ * with a_pq = real(a, REAL32).  x is overwritten (in: r, out: z).       */
+void orc_dilu_apply(const orc_grid *g, const double *a, const float *dinv, float *x) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int A = nr - 2, B = nt - 2, C = np - 2;
+  const size_t N = (size_t)A * B * C;
+  const size_t sj = (size_t)A, sk = (size_t)A * B;
+  for (int k = 0; k < C; k++)
+    for (int j = 0; j < B; j++)
+      for (int i = 0; i < A; i++) {
+        const size_t p = i + sj * j + sk * k;
+        float y = x[p];
+        if (k == C - 1) y = y - (float)a[6 * N + p] * (dinv[i + sj * j] * x[i + sj * j]);
+        if (k > 0) y = y - (float)a[0 * N + p] * (dinv[p - sk] * x[p - sk]);
+        if (j > 0) y = y - (float)a[1 * N + p] * (dinv[p - sj] * x[p - sj]);
+        if (i > 0) y = y - (float)a[2 * N + p] * (dinv[p - 1] * x[p - 1]);
+        x[p] = y;
+      }
+  for (int k = C - 1; k >= 0; k--)
+    for (int j = B - 1; j >= 0; j--)
+      for (int i = A - 1; i >= 0; i--) {
+        const size_t p = i + sj * j + sk * k;
+        float s = x[p];
+        if (i < A - 1) s = s - (float)a[4 * N + p] * x[p + 1];
+        if (j < B - 1) s = s - (float)a[5 * N + p] * x[p + sj];
+        if (k < C - 1) s = s - (float)a[6 * N + p] * x[p + sk];
+        if (k == 0) s = s - (float)a[0 * N + p] * x[i + sj * j + sk * (C - 1)];
+        x[p] = s * dinv[p];
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* cgsolve + ernorm: src/pot3d.F90:4231-4485.
+ * ifprec: 0 none, 1 diagonal, 2 ILU0 via diacsr+ilu0+lusol (the
+ * reference's CPU path, :4684-4709, :5202-5220), 3 D-ILU (FP64 pivots,
+ * FP32 apply; the product's ILU0 arithmetic).
+ * x: initial guess in, solution out.  r: rhs in, residual out.
+ * hist (optional, length nhist): epsn recorded at every iteration.
+ * Returns ierr (0 converged, 1 ncgmax reached); *ncg_out, *epsn_out.     */
+typedef struct {
+  int ifprec;
+  const orc_grid *g;
+  const double *a;
+  float *a_i;                    /* ifprec 1 */
+  float *lu, *lud_i;             /* ifprec 2 */
+  int *luja, *n1, *n2;
+  float *dinv;                   /* ifprec 3 */
+  float *x32;
+} orc_prec;
+
+static void prec_apply(orc_prec *P, double *x, size_t N) {
+  if (P->ifprec == 1) {
+    for (size_t n = 0; n < N; n++) x[n] = (double)P->a_i[n] * x[n];
+  } else if (P->ifprec == 2) {
+    for (size_t n = 0; n < N; n++) P->x32[n] = (float)x[n];
+    orc_lusol((int64_t)N, P->x32, P->lu, P->luja, P->n1, P->n2, P->lud_i);
+    for (size_t n = 0; n < N; n++) x[n] = (double)P->x32[n];
+  } else if (P->ifprec == 3) {
+    for (size_t n = 0; n < N; n++) P->x32[n] = (float)x[n];
+    orc_dilu_apply(P->g, P->a, P->dinv, P->x32);
+    for (size_t n = 0; n < N; n++) x[n] = (double)P->x32[n];
+  }
+}
+
+static int prec_setup(orc_prec *P, const orc_grid *g, const double *a, int ifprec) {
+  const size_t N = (size_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  memset(P, 0, sizeof(*P));
+  P->ifprec = ifprec;
+  P->g = g;
+  P->a = a;
+  if (ifprec == 1) {
+    P->a_i = (float *)malloc(N * sizeof(float));
+    orc_diag_prec(g, a, P->a_i);
+  } else if (ifprec == 2) {
+    int *IA = (int *)malloc((N + 1) * sizeof(int));
+    const int64_t M = orc_getM(g, IA);
+    float *Av = (float *)malloc((size_t)M * sizeof(float));
+    int *JA = (int *)malloc((size_t)M * sizeof(int));
+    int *dptr = (int *)malloc(N * sizeof(int));
+    orc_diacsr(g, a, IA, Av, JA, dptr);
+    int icode = orc_ilu0((int64_t)N, Av, JA, IA, dptr);
+    P->lu = (float *)malloc((size_t)M * sizeof(float));
+    P->luja = (int *)malloc((size_t)M * sizeof(int));
+    P->n1 = (int *)malloc(N * sizeof(int));
+    P->n2 = (int *)malloc(N * sizeof(int));
+    P->lud_i = (float *)malloc(N * sizeof(float));
+    orc_lu2luopt((int64_t)N, P->lu, Av, IA, JA, P->luja, dptr, P->n1, P->n2);
+    for (size_t n = 0; n < N; n++) P->lud_i[n] = (float)(1.0 / (double)Av[dptr[n] - 1]);
+    free(IA);
+    free(Av);
+    free(JA);
+    free(dptr);
+    P->x32 = (float *)malloc(N * sizeof(float));
+    if (icode != 0) return icode; /* reference prints and continues */
+  } else if (ifprec == 3) {
+    double *d = (double *)malloc(N * sizeof(double));
+    orc_dilu_pivots(g, a, d);
+    P->dinv = (float *)malloc(N * sizeof(float));
+    for (size_t n = 0; n < N; n++) P->dinv[n] = (float)(1.0 / d[n]);
+    free(d);
+    P->x32 = (float *)malloc(N * sizeof(float));
+  }
+  return 0;
+}
+
+static void prec_free(orc_prec *P) {
+  free(P->a_i);
+  free(P->lu);
+  free(P->lud_i);
+  free(P->luja);
+  free(P->n1);
+  free(P->n2);
+  free(P->dinv);
+  free(P->x32);
+}
+
+/* Apply the chosen preconditioner once (for per-routine parity tests). */
+int orc_prec_apply(const orc_grid *g, const double *a, int ifprec, double *x) {
+  orc_prec P;
+  const size_t N = (size_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  int rc = prec_setup(&P, g, a, ifprec);
+  prec_apply(&P, x, N);
+  prec_free(&P);
+  return rc;
+}
+
+static double ernorm(double bdotb, double rdotr) {
+  const double epssq = rdotr / bdotb;
+  return copysign(1.0, epssq) * sqrt(fabs(epssq));
+}
+
+int orc_cgsolve(const orc_grid *g, const double *a, int ifprec, double epscg, int ncgmax,
+                double *x, double *r, int *ncg_out, double *epsn_out, double *hist, int nhist) {
+  const size_t N = (size_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  const size_t tot = (size_t)g->nr * g->nt * g->np;
+  double *p = (double *)malloc(N * sizeof(double));
+  double *ap = (double *)malloc(N * sizeof(double));
+  double *xa = (double *)calloc(tot, sizeof(double));
+  orc_prec P;
+  prec_setup(&P, g, a, ifprec);
+  int ncg = 0, ierr = -1;
+  double epsn = 0.0;
+  memcpy(p, r, N * sizeof(double));
+  prec_apply(&P, p, N);
+  const double bdotb = det_dot(r, p, N);
+  if (bdotb == 0.0) {
+    memset(x, 0, N * sizeof(double));
+    ierr = 0;
+    epsn = 0.0;
+    goto done;
+  }
+  orc_ax(g, a, x, ap, xa);
+  for (size_t n = 0; n < N; n++) {
+    r[n] = r[n] - ap[n];
+    p[n] = r[n];
+  }
+  prec_apply(&P, p, N);
+  double rdotr = det_dot(r, p, N);
+  epsn = ernorm(bdotb, rdotr);
+  if (hist && nhist > 0) hist[0] = epsn;
+  if (epsn < epscg) {
+    ierr = 0;
+    goto done;
+  } else if (ncg >= ncgmax) {
+    ierr = 1;
+    goto done;
+  }
+  for (;;) {
+    ncg++;
+    orc_ax(g, a, p, ap, xa);
+    const double pdotap = det_dot(p, ap, N);
+    const double alphai = rdotr / pdotap;
+    long n;
+#pragma omp parallel for schedule(static)
+    for (n = 0; n < (long)N; n++) {
+      x[n] = x[n] + alphai * p[n];
+      r[n] = r[n] - alphai * ap[n];
+      ap[n] = r[n];
+    }
+    prec_apply(&P, ap, N);
+    const double rdotr_old = rdotr;
+    rdotr = det_dot(r, ap, N);
+    epsn = ernorm(bdotb, rdotr);
+    if (hist && ncg < nhist) hist[ncg] = epsn;
+    if (epsn < epscg) {
+      ierr = 0;
+      break;
+    } else if (ncg >= ncgmax) {
+      ierr = 1;
+      break;
+    }
+    const double betai = rdotr / rdotr_old;
+#pragma omp parallel for schedule(static)
+    for (n = 0; n < (long)N; n++) p[n] = betai * p[n] + ap[n];
+  }
+done:
+  prec_free(&P);
+  free(p);
+  free(ap);
+  free(xa);
+  *ncg_out = ncg;
+  *epsn_out = epsn;
+  return ierr;
+}
+
+/* ---------------------------------------------------------------------- */
+/* getb: src/pot3d.F90:6187-6233.  br(nr-1,nt,np), bt(nr,nt-1,np),
+ * bp(nr,nt,np-1), Fortran order.                                         */
+void orc_getb(const orc_grid *g, const double *phi, double *br, double *bt, double *bp) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int nrm1 = nr - 1, ntm1 = nt - 1, npm1 = np - 1;
+  for (int k = 1; k <= np; k++)
+    for (int j = 1; j <= nt; j++)
+      for (int i = 1; i <= nrm1; i++)
+        br[(i - 1) + (size_t)nrm1 * ((j - 1) + (size_t)nt * (k - 1))] =
+            (phi[IDX3(i + 1, j, k)] - phi[IDX3(i, j, k)]) / g->dr[i - 1];
+  for (int k = 1; k <= np; k++)
+    for (int j = 1; j <= ntm1; j++)
+      for (int i = 1; i <= nr; i++)
+        bt[(i - 1) + (size_t)nr * ((j - 1) + (size_t)ntm1 * (k - 1))] =
+            (phi[IDX3(i, j + 1, k)] - phi[IDX3(i, j, k)]) / (g->rh[i - 1] * g->dt[j - 1]);
+  for (int k = 1; k <= npm1; k++)
+    for (int j = 1; j <= nt; j++)
+      for (int i = 1; i <= nr; i++)
+        bp[(i - 1) + (size_t)nr * ((j - 1) + (size_t)nt * (k - 1))] =
+            (phi[IDX3(i, j, k + 1)] - phi[IDX3(i, j, k)]) /
+            (g->rh[i - 1] * g->sth[j - 1] * g->dp[k - 1]);
+}
+
+/* magnetic_energy: src/pot3d.F90:6235-6321.  w[0..2] = Br^2, Bt^2, Bp^2. */
+void orc_energy(const orc_grid *g, const double *br, const double *bt, const double *bp,
+                double *w) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  const int nrm1 = nr - 1, ntm1 = nt - 1;
+  double wr = 0.0, wt = 0.0, wp = 0.0;
+#define BR(i, j, k) br[((i)-1) + (size_t)nrm1 * (((j)-1) + (size_t)nt * ((k)-1))]
+#define BT(i, j, k) bt[((i)-1) + (size_t)nr * (((j)-1) + (size_t)ntm1 * ((k)-1))]
+#define BP(i, j, k) bp[((i)-1) + (size_t)nr * (((j)-1) + (size_t)nt * ((k)-1))]
+  for (int k = 2; k <= np - 1; k++)
+    for (int j = 2; j <= nt - 1; j++)
+      for (int i = 2; i <= nr - 1; i++) {
+        const double rh = g->rh[i - 1];
+        const double dv = rh * rh * g->drh[i - 1] * g->dth[j - 1] * g->sth[j - 1] * g->dph[k - 1];
+        const double brav = 0.5 * (BR(i, j, k) + BR(i - 1, j, k));
+        const double btav = 0.5 * (BT(i, j, k) + BT(i, j - 1, k));
+        const double bpav = 0.5 * (BP(i, j, k) + BP(i, j, k - 1));
+        wr = wr + 0.5 * (brav * brav) * dv;
+        wt = wt + 0.5 * (btav * btav) * dv;
+        wp = wp + 0.5 * (bpav * bpav) * dv;
+      }
+#undef BR
+#undef BT
+#undef BP
+  w[0] = wr;
+  w[1] = wt;
+  w[2] = wp;
+}
+
+/* One call: phi -> energies (getb + magnetic_energy). */
+void orc_phi_energy(const orc_grid *g, const double *phi, double *w) {
+  const int nr = g->nr, nt = g->nt, np = g->np;
+  double *br = (double *)malloc(sizeof(double) * (size_t)(nr - 1) * nt * np);
+  double *bt = (double *)malloc(sizeof(double) * (size_t)nr * (nt - 1) * np);
+  double *bp = (double *)malloc(sizeof(double) * (size_t)nr * nt * (np - 1));
+  orc_getb(g, phi, br, bt, bp);
+  orc_energy(g, br, bt, bp, w);
+  free(br);
+  free(bt);
+  free(bp);
+}

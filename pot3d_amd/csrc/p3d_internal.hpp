@@ -1,0 +1,130 @@
+// Internal definitions shared by the gfx950 kernels and the host side of
+// libpot3d_hip.so.  Layout conventions (identical to the reference's packed
+// solver vectors, src/pot3d.F90:3907-3917, 5323):
+//   interior grid a=nr-2 (r), b=nt-2 (theta), c=np-2 (phi), N=a*b*c
+//   packed index n = i + a*(j + b*k), 0-based i<a, j<b, k<c (r fastest)
+// All 1-D metric factors live in one device table (MetricTables).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p3d {
+
+// Separable stencil factors (FP64).  For interior point (i,j,k):
+//   a1 = -(DRH[i]*CP[j]) *DPL[k]   coupling to (i,j,k-1)
+//   a7 = -(DRH[i]*CP[j]) *DPU[k]   coupling to (i,j,k+1)
+//   a2 = -(DRH[i]*DPH[k])*CTL[j]   coupling to (i,j-1,k)
+//   a6 = -(DRH[i]*DPH[k])*CTH[j]   coupling to (i,j+1,k)
+//   a3 = -(SD[j]*DPH[k]) *CRL[i]   coupling to (i-1,j,k)
+//   a5 = -(SD[j]*DPH[k]) *CRH[i]   coupling to (i+1,j,k)
+//   a4 = -(a1+a2+a3+a5+a6+a7)
+// with DRH=drh(i), CRL=r(i-1)^2/dr(i-1), CRH=r(i)^2/dr(i), SD=sth*dth,
+// CP=dth/sth, CTL=st(j-1)/dt(j-1), CTH=st(j)/dt(j), DPH=dph(k),
+// DPL=1/dp(k-1), DPU=1/dp(k)  -- the factors of load_matrix_pot3d_solve
+// (src/pot3d.F90:4590-4611).  Because CRH[i]==CRL[i+1], CTH[j]==CTL[j+1]
+// and DPU[k]==DPL[k+1], a5(i)==a3(i+1), a6(j)==a2(j+1), a7(k)==a1(k+1)
+// bitwise: the operator is exactly symmetric, as the reference's is.
+struct Metric {
+  const double* DRH;  // a
+  const double* CRL;  // a
+  const double* CRH;  // a
+  const double* SD;   // b
+  const double* CP;   // b
+  const double* CTL;  // b
+  const double* CTH;  // b
+  const double* DPH;  // c
+  const double* DPL;  // c
+  const double* DPU;  // c
+  const double* PW;   // c: dph(k)/(2 pi), pole-average weights
+  int a, b, c;
+  double pm_r1;       // r1 ghost factor (+1 Neumann, -1 Dirichlet)
+};
+
+struct Coef {
+  double a1, a2, a3, a4, a5, a6, a7;
+};
+
+__device__ __forceinline__ Coef coef_at(const Metric& m, int i, int j, int k) {
+  Coef c;
+  const double drh = m.DRH[i];
+  const double p1 = drh * m.CP[j];
+  const double dph = m.DPH[k];
+  const double p2 = drh * dph;
+  const double p3 = m.SD[j] * dph;
+  c.a1 = -(p1 * m.DPL[k]);
+  c.a7 = -(p1 * m.DPU[k]);
+  c.a2 = -(p2 * m.CTL[j]);
+  c.a6 = -(p2 * m.CTH[j]);
+  c.a3 = -(p3 * m.CRL[i]);
+  c.a5 = -(p3 * m.CRH[i]);
+  c.a4 = -(c.a1 + c.a2 + c.a3 + c.a5 + c.a6 + c.a7);
+  return c;
+}
+
+// n -> (i,j,k) with 32-bit magic-number division, exact for n < 2^31
+// (shift = ceil(log2 d), mul = floor(2^32 (2^shift - d)/d) + 1).
+struct FastDiv {
+  uint32_t d, mul, shift;
+};
+
+__host__ inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  f.shift = s;
+  uint64_t one = 1;
+  f.mul = (uint32_t)(((one << 32) * ((one << s) - d)) / d + 1);
+  return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return (__umulhi(n, f.mul) + n) >> f.shift;
+}
+
+struct Grid3 {
+  int a, b, c;
+  uint32_t N, ab;
+  FastDiv da, dab;
+};
+
+__device__ __forceinline__ void unflatten(const Grid3& g, uint32_t n, int& i, int& j, int& k) {
+  uint32_t kk = fdiv(n, g.dab);
+  uint32_t rem = n - kk * g.ab;
+  uint32_t jj = fdiv(rem, g.da);
+  i = (int)(rem - jj * (uint32_t)g.a);
+  j = (int)jj;
+  k = (int)kk;
+}
+
+// Device-side solver state (one per solve, in device memory).
+struct CgState {
+  double bdotb;      // b . M^-1 b
+  double rdotr;      // current r . z
+  double rdotr_old;
+  double pdotap;
+  double alpha;
+  double beta;
+  double epsn;       // sign(rdotr/bdotb)*sqrt(|rdotr/bdotb|)
+  double rr;         // r . r (two-norm residual squared)
+  double bb;         // b . b
+  double relres;     // sqrt(rr/bb)
+  double epscg;
+  double* hist;      // epsn per iteration (index ncg), may be null
+  int hist_cap;
+  int ncg;
+  int ncgmax;
+  int done;          // 0 running, 1 converged, 2 ncgmax reached,
+                     // 3 non-finite residual, 4 zero right-hand side
+  int stop_mode;     // 0: epsn (reference), 1: ||r||2/||b||2
+  int pad_;
+};
+
+// Reduction slots used by the "last block finishes" pattern.
+struct Reduce {
+  double* partial;   // [3 * max_blocks]
+  unsigned int* counter;  // one counter per reduction site
+  int max_blocks;
+};
+
+}  // namespace p3d

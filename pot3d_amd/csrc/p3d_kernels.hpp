@@ -1,0 +1,131 @@
+// Host-side launchers of the gfx950 kernels (p3d_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "p3d_internal.hpp"
+
+namespace p3d {
+
+enum Prec { PREC_NONE = 0, PREC_DIAG = 1, PREC_ILU = 2 };
+
+// Epilogues run by the last block of a fused reduction (one thread).
+enum Epi {
+  EPI_NONE = 0,
+  EPI_BDOTB = 1,   // s0=b.z, s1=b.b        -> bdotb, bb (zero rhs -> done=4)
+  EPI_RINIT = 2,   // s0=r.z, s1=r.r        -> rdotr, epsn, convergence at ncg=0
+  EPI_PAP = 3,     // s0=p.ap               -> ncg++, pdotap, alpha
+  EPI_RITER = 4,   // s0=r.z, s1=r.r        -> rdotr, epsn, convergence, beta
+  EPI_RR = 5,      // s1=r.r                -> rr (ILU path; r.z comes later)
+  EPI_RZITER = 6,  // s0=r.z (rr from EPI_RR) -> as EPI_RITER
+  EPI_STORE = 7,   // s0,s1 -> out[0], out[1] (plain dot products)
+};
+
+struct Launch {
+  int G;        // blocks of a streaming kernel (multiple of 8)
+  int CH;       // points per block (multiple of 256)
+};
+
+constexpr int kBlock = 256;
+constexpr int kPoleSegs = 16;
+
+Launch plan_stream(uint32_t N, int max_blocks);
+
+// Pole averages (m=0 component) of x at j=0 and j=b-1, per interior i, as
+// kPoleSegs partial sums each: part[(pole*kPoleSegs+seg)*a + i].
+template <typename T>
+void launch_pole_sums(const Metric& m, const Grid3& g, const T* x, double* part, hipStream_t s);
+
+// y = A x (homogeneous boundary values; mode 0) or y = b - A x (mode 1).
+// Reduces s0 = x.y (mode 0) or s0 = y.y (mode 1) and runs epilogue `epi`.
+template <typename T>
+void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x, T* y,
+                    const T* b, int mode, const double* pole_part, CgState* st, Reduce red,
+                    int slot, Epi epi, double* out, bool guard, hipStream_t s);
+
+// CG update x += alpha p, r -= alpha ap, and for PREC_DIAG/NONE the reduction
+// of r.z and r.r with epilogue EPI_RITER; for PREC_ILU only r.r (EPI_RR).
+template <typename T>
+void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
+                      const T* p, const T* ap, CgState* st, Reduce red, int slot,
+                      hipStream_t s);
+
+// p = z + beta p (z = M^-1 r computed in place for DIAG/NONE, read for ILU).
+template <typename T>
+void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
+                     const T* r, const T* z, CgState* st, hipStream_t s);
+
+// Generic fused dot: s0 = x.y, s1 = u.v (v/u may be null), epilogue epi.
+template <typename T>
+void launch_dot2(const Grid3& g, const Launch& L, const T* x, const T* y, const T* u,
+                 const T* v, CgState* st, Reduce red, int slot, Epi epi, double* out,
+                 bool guard, hipStream_t s);
+
+// z = M^-1 r for DIAG/NONE (r,z may alias), with optional s0=r.z, s1=r.r.
+template <typename T>
+void launch_point_prec(const Metric& m, const Grid3& g, const Launch& L, int prec, const T* r,
+                       T* z, CgState* st, Reduce red, int slot, Epi epi, double* out,
+                       hipStream_t s);
+
+// Residual init: r = r - ap, and p = z later.
+template <typename T>
+void launch_sub(const Grid3& g, const Launch& L, T* r, const T* ap, hipStream_t s);
+
+template <typename T>
+void launch_copy(const Grid3& g, const Launch& L, T* dst, const T* src, hipStream_t s);
+
+template <typename TD, typename TS>
+void launch_cast_scale(uint32_t N, TD* dst, const TS* src, const double* scale_num,
+                       const double* scale_den, int op, hipStream_t s);
+
+void launch_fill(double* p, double v, size_t n, hipStream_t s);
+void launch_fill_f(float* p, float v, size_t n, hipStream_t s);
+
+// RHS from the boundary data (closed form of the boundary trick).
+void launch_rhs(const Metric& m, const Grid3& g, const double* br0, const double* br1,
+                double dr1, double drn, int nt, double* rhs, hipStream_t s);
+
+// D-ILU pivots (FP64, hyperplane level launches) -> dinv (FP32).
+void launch_dilu_factor(const Metric& m, const Grid3& g, double* d, float* dinv,
+                        hipStream_t s);
+
+// Stored couplings of the legacy CSR input (FP32, N each unless noted):
+// lo_k/lo_j/lo_i couple p to (i,j,k-1)/(i,j-1,k)/(i-1,j,k), up_i/up_j/up_k
+// to (i+1,j,k)/(i,j+1,k)/(i,j,k+1); zero where the reference drops the
+// coupling.  wrap_lo[i+a*j]: (i,j,c-1)->(i,j,0); wrap_up[i+a*j]: (i,j,0)->
+// (i,j,c-1).
+struct CoefArrays {
+  const float* lo_k;
+  const float* lo_j;
+  const float* lo_i;
+  const float* up_i;
+  const float* up_j;
+  const float* up_k;
+  const float* wrap_lo;
+  const float* wrap_up;
+};
+
+// Level-scheduled D-ILU sweeps: r (T) -> z (T); yv is FP32 scratch of N.
+template <typename T>
+void launch_dilu_solve_levels(const Metric& m, const Grid3& g, const float* dinv, const T* r,
+                              float* yv, T* z, const CoefArrays* ca, bool wrap,
+                              const CgState* guard, hipStream_t s);
+
+void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid3& g, bool wrap,
+                               double* d, float* dinv, hipStream_t s);
+
+// Post-solve phi (nr,nt,np) from packed x: unpack, boundary values with
+// vmask=1, pole averages, phi seam.  sums: device scratch of 2*nr doubles.
+void launch_fill_phi(const Metric& m, const Grid3& g, int nr, int nt, int np, const double* x,
+                     const double* br0, const double* br1, double dr1, double drn,
+                     const double* dph_full, double* phi, double* sums, hipStream_t s);
+
+// Magnetic energy of phi (getb fused with magnetic_energy).
+struct EnergyMesh {
+  const double *dr, *rh, *drh, *dt, *dth, *sth, *dp, *dph;
+};
+void launch_energy(const EnergyMesh& em, int nr, int nt, int np, const double* phi,
+                   double* partial, unsigned* counter, double* out3, int G, hipStream_t s);
+void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi, double* br,
+                 double* bt, double* bp, hipStream_t s);
+
+}  // namespace p3d

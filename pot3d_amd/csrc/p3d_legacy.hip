@@ -1,0 +1,301 @@
+// Legacy external-LU-solver ABI (include/pot3d_hip.h, group 1): the three
+// symbols POT3D binds under -DHIPSPARSE (src/pot3d.F90:581-609), replacing
+// src/lusol_hipsparse.c.
+//
+// The caller's CSR is the diacsr matrix (src/pot3d.F90:4870-5018): the
+// 7-point stencil with r/theta boundary couplings dropped and, at one rank
+// in phi, the periodic wrap kept.  The structure is recognised from IA/JA
+// (grid a x b x c, wrap on/off); the couplings are split into stored
+// planes and factorised as D-ILU (FP64 pivots), which equals ILU(0) on this
+// pattern.  Anything else is rejected with exit(1), as the reference shim
+// does on any sparse-library error.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/pot3d_hip.h"
+#include "p3d_kernels.hpp"
+
+using namespace p3d;
+
+// OpenMP mapping lookup, resolved only when the host program uses OpenMP
+// offload (the -DHIPSPARSE convention passes host addresses of mapped
+// arrays, src/lusol_hipsparse.c:75-79).
+extern "C" void* omp_get_mapped_ptr(const void* ptr, int device_num) __attribute__((weak));
+extern "C" int omp_get_default_device(void) __attribute__((weak));
+
+namespace {
+
+struct Legacy {
+  bool loaded = false;
+  int N = 0, a = 0, b = 0, c = 0;
+  bool wrap = true;
+  Grid3 g;
+  hipStream_t stream = nullptr;
+  float* planes = nullptr;  // 6*N couplings + 2*a*b wrap
+  CoefArrays ca;
+  float* dinv = nullptr;
+  float* yv = nullptr;
+  double* xbuf = nullptr;   // staging for host x
+};
+
+Legacy G;
+
+[[noreturn]] void die(const char* msg) {
+  fprintf(stderr, " ERROR! external_lu_solver: %s\n", msg);
+  fflush(stderr);
+  exit(1);
+}
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, " ERROR! external_lu_solver: %s: %s\n", what, hipGetErrorString(e));
+    fflush(stderr);
+    exit(1);
+  }
+}
+
+bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// Resolve a caller pointer: an OpenMP-mapped host address -> its device
+// address; a device address -> itself; else nullptr (plain host memory).
+void* device_view(void* p) {
+  if (omp_get_mapped_ptr && omp_get_default_device) {
+    void* d = omp_get_mapped_ptr(p, omp_get_default_device());
+    if (d) return d;
+  }
+  return is_device_ptr(p) ? p : nullptr;
+}
+
+template <typename V>
+void fetch(std::vector<V>& dst, const V* src, size_t n) {
+  dst.resize(n);
+  void* dv = device_view((void*)src);
+  if (dv)
+    check(hipMemcpy(dst.data(), dv, n * sizeof(V), hipMemcpyDeviceToHost), "copy CSR to host");
+  else
+    memcpy(dst.data(), src, n * sizeof(V));
+}
+
+// Expected columns (1-based, in diacsr order) of row p=(i,j,k).
+int expected_row(int i, int j, int k, int a, int b, int c, bool wrap, long long* cols) {
+  const long long N = (long long)a * b * c, ab = (long long)a * b;
+  const long long p = (long long)i + (long long)a * j + ab * k + 1;
+  const long long off[7] = {-ab, -a, -1, 0, 1, a, ab};
+  bool ok[7] = {true, true, true, true, true, true, true};
+  if (i == 0) ok[2] = false;
+  if (i == a - 1) ok[4] = false;
+  if (j == 0) ok[1] = false;
+  if (j == b - 1) ok[5] = false;
+  if (!wrap) {
+    if (k == 0) ok[0] = false;
+    if (k == c - 1) ok[6] = false;
+  }
+  int n = 0;
+  for (int q = 0; q < 7; q++)
+    if (ok[q] && p + off[q] > N) cols[n++] = p + off[q] - N;
+  for (int q = 0; q < 7; q++)
+    if (ok[q] && p + off[q] >= 1 && p + off[q] <= N) cols[n++] = p + off[q];
+  for (int q = 0; q < 7; q++)
+    if (ok[q] && p + off[q] < 1) cols[n++] = p + off[q] + N;
+  return n;
+}
+
+bool match_pattern(const std::vector<int>& IA, const std::vector<int>& JA, int a, int b, int c,
+                   bool wrap) {
+  const long long N = (long long)a * b * c;
+  if (IA[0] != 1) return false;
+  long long cols[7];
+  long long p = 0;
+  for (int k = 0; k < c; k++)
+    for (int j = 0; j < b; j++)
+      for (int i = 0; i < a; i++, p++) {
+        const int n = expected_row(i, j, k, a, b, c, wrap, cols);
+        if (IA[p + 1] - IA[p] != n) return false;
+        for (int q = 0; q < n; q++)
+          if (JA[IA[p] - 1 + q] != cols[q]) return false;
+      }
+  return p == N;
+}
+
+void unload() {
+  if (!G.loaded) return;
+  if (G.stream) hipStreamSynchronize(G.stream);
+  if (G.planes) hipFree(G.planes);
+  if (G.dinv) hipFree(G.dinv);
+  if (G.yv) hipFree(G.yv);
+  if (G.xbuf) hipFree(G.xbuf);
+  if (G.stream) hipStreamDestroy(G.stream);
+  G = Legacy();
+}
+
+}  // namespace
+
+extern "C" {
+
+void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M) {
+  if (G.loaded) unload();
+  if (N < 1 || M < 1) die("invalid matrix size");
+  std::vector<int> IA, JA;
+  std::vector<float> AV;
+  fetch(IA, csr_i, (size_t)N + 1);
+  const long long nnz = (long long)IA[N] - 1;
+  if (nnz < N || nnz > (long long)M) die("row pointer array inconsistent with M");
+  fetch(JA, csr_j, (size_t)nnz);
+  fetch(AV, csr_lu, (size_t)nnz);
+  // grid from row 1: off-diagonal columns 2, 1+a, 1+ab [, 1+(c-1)ab]
+  std::vector<long long> offs;
+  for (int e = IA[0] - 1; e < IA[1] - 1; e++)
+    if (JA[e] != 1) offs.push_back(JA[e] - 1);
+  bool found = false;
+  int a = 0, b = 0, c = 0;
+  bool wrap = true;
+  if (offs.size() >= 3 && offs[0] == 1) {
+    const long long A = offs[1], AB = offs[2];
+    if (A >= 2 && AB % A == 0 && (long long)N % AB == 0) {
+      a = (int)A;
+      b = (int)(AB / A);
+      c = (int)(N / AB);
+      for (int w = 1; w >= 0 && !found; w--) {
+        if (c >= 8 && match_pattern(IA, JA, a, b, c, w == 1)) {
+          found = true;
+          wrap = (w == 1);
+        }
+      }
+    }
+  }
+  if (!found) die("matrix is not the POT3D 7-point pattern (unsupported by this solver)");
+  G.N = N;
+  G.a = a;
+  G.b = b;
+  G.c = c;
+  G.wrap = wrap;
+  G.g.a = a;
+  G.g.b = b;
+  G.g.c = c;
+  G.g.N = (uint32_t)N;
+  G.g.ab = (uint32_t)(a * b);
+  G.g.da = make_fastdiv((uint32_t)a);
+  G.g.dab = make_fastdiv((uint32_t)(a * b));
+  // split into planes
+  const size_t n = (size_t)N, ab = (size_t)a * b;
+  std::vector<float> P(7 * n + 2 * ab, 0.0f);  // 6 couplings + diag + 2 wrap planes
+  float *lo_k = P.data(), *lo_j = lo_k + n, *lo_i = lo_j + n, *up_i = lo_i + n,
+        *up_j = up_i + n, *up_k = up_j + n, *dg = up_k + n, *w_lo = dg + n, *w_up = w_lo + ab;
+  std::vector<int> role(nnz);  // 0..6 as planes, 7 wrap_lo, 8 wrap_up
+  for (size_t p = 0; p < n; p++) {
+    const int k = (int)(p / ab);
+    const size_t ij = p - (size_t)k * ab;
+    for (int e = IA[p] - 1; e < IA[p + 1] - 1; e++) {
+      const long long q = (long long)JA[e] - 1;
+      const long long d = q - (long long)p;
+      const float v = AV[e];
+      int rl;
+      if (d == 0) {
+        dg[p] = v;
+        rl = 6;
+      } else if (d == -1) {
+        lo_i[p] = v;
+        rl = 2;
+      } else if (d == 1) {
+        up_i[p] = v;
+        rl = 3;
+      } else if (d == -(long long)a) {
+        lo_j[p] = v;
+        rl = 1;
+      } else if (d == (long long)a) {
+        up_j[p] = v;
+        rl = 4;
+      } else if (k == c - 1 && q == (long long)ij) {
+        w_lo[ij] = v;  // wrap to (i,j,0), lower (checked before the k-1 test)
+        rl = 7;
+      } else if (k == 0 && q == (long long)(ij + (size_t)(c - 1) * ab)) {
+        w_up[ij] = v;
+        rl = 8;
+      } else if (d == -(long long)ab) {
+        lo_k[p] = v;
+        rl = 0;
+      } else {
+        up_k[p] = v;
+        rl = 5;
+      }
+      role[e] = rl;
+    }
+  }
+  check(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking), "stream");
+  check(hipMalloc((void**)&G.planes, P.size() * sizeof(float)), "allocate couplings");
+  check(hipMemcpy(G.planes, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice),
+        "copy couplings");
+  float* dP = G.planes;
+  G.ca.lo_k = dP;
+  G.ca.lo_j = dP + n;
+  G.ca.lo_i = dP + 2 * n;
+  G.ca.up_i = dP + 3 * n;
+  G.ca.up_j = dP + 4 * n;
+  G.ca.up_k = dP + 5 * n;
+  const float* d_dg = dP + 6 * n;
+  G.ca.wrap_lo = dP + 7 * n;
+  G.ca.wrap_up = dP + 7 * n + ab;
+  check(hipMalloc((void**)&G.dinv, n * sizeof(float)), "allocate pivots");
+  check(hipMalloc((void**)&G.yv, n * sizeof(float)), "allocate work");
+  double* dpiv = nullptr;
+  check(hipMalloc((void**)&dpiv, n * sizeof(double)), "allocate pivots");
+  launch_dilu_factor_stored(G.ca, d_dg, G.g, wrap, dpiv, G.dinv, G.stream);
+  check(hipStreamSynchronize(G.stream), "D-ILU factorisation");
+  // zero-pivot check and in-place LU write-back (L: a_pq/d_q, U: a_pq,
+  // diagonal: d_p), as csrilu02 leaves the caller's values.
+  std::vector<double> d(n);
+  check(hipMemcpy(d.data(), dpiv, n * sizeof(double), hipMemcpyDeviceToHost), "copy pivots");
+  hipFree(dpiv);
+  for (size_t p = 0; p < n; p++)
+    if (d[p] == 0.0) {
+      fprintf(stderr, " ERROR! ILU0 zero pivot at row %zu\n", p + 1);
+      exit(1);
+    }
+  for (size_t p = 0; p < n; p++) {
+    for (int e = IA[p] - 1; e < IA[p + 1] - 1; e++) {
+      const long long q = (long long)JA[e] - 1;
+      if (role[e] == 6)
+        AV[e] = (float)d[p];
+      else if (q < (long long)p)
+        AV[e] = (float)((double)AV[e] / d[q]);
+    }
+  }
+  void* dv = device_view(csr_lu);
+  if (dv)
+    check(hipMemcpy(dv, AV.data(), nnz * sizeof(float), hipMemcpyHostToDevice), "write LU");
+  else
+    memcpy(csr_lu, AV.data(), nnz * sizeof(float));
+  G.loaded = true;
+}
+
+void external_lu_solver(double* x) {
+  if (!G.loaded) die("external_lu_solver called before load_external_lu_solver");
+  double* dx = (double*)device_view(x);
+  const size_t nb = (size_t)G.N * sizeof(double);
+  if (!dx) {
+    if (!G.xbuf) check(hipMalloc((void**)&G.xbuf, nb), "allocate x");
+    check(hipMemcpyAsync(G.xbuf, x, nb, hipMemcpyHostToDevice, G.stream), "copy x");
+    dx = G.xbuf;
+  }
+  Metric m;
+  memset(&m, 0, sizeof m);
+  launch_dilu_solve_levels<double>(m, G.g, G.dinv, dx, G.yv, dx, &G.ca, G.wrap, nullptr,
+                                   G.stream);
+  check(hipGetLastError(), "trisolve launch");
+  if (dx == G.xbuf) check(hipMemcpyAsync(x, G.xbuf, nb, hipMemcpyDeviceToHost, G.stream), "copy x");
+  check(hipStreamSynchronize(G.stream), "trisolve");
+}
+
+void unload_external_lu_solver(void) { unload(); }
+
+}  // extern "C"

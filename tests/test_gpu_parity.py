@@ -1,0 +1,216 @@
+"""GPU parity: every operation of libpot3d_hip.so against the CPU oracle on
+the same seeded inputs, through the C ABI.
+
+Tolerances: FP64 operator/rhs/phi/energy to 1e-13 relative to the natural
+scale (the GPU computes the coefficients from separable factor tables and
+may contract products into FMAs, so last-bit differences are expected);
+FP32 preconditioners to 2e-5 relative; solves to the north-star bar
+(potential 1e-8 relative, energy 1e-7).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import random_br0, small_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def solver_mod():
+    from pot3d_amd import solver
+    return solver
+
+
+MESHES = {
+    "tiny": dict(nr=9, nt=8, np_=12),
+    "odd": dict(nr=13, nt=11, np_=18, drratio=2.0),
+    "mid": dict(nr=33, nt=45, np_=70),
+}
+
+
+def _mesh(name):
+    return small_mesh(**MESHES[name])
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+@pytest.mark.parametrize("pm_r1", [-1.0, 1.0])
+def test_operator(oracle_mod, solver_mod, name, pm_r1):
+    say(f"[gpu] operator mesh={name} pm_r1={pm_r1}")
+    mesh = _mesh(name)
+    o = oracle_mod.Oracle(mesh, pm_r1)
+    with solver_mod.Pot3dSolver(mesh, pm_r1, 1) as s:
+        rng = np.random.default_rng(7)
+        for _ in range(2):
+            x = rng.standard_normal(o.N)
+            y = s.ax(x)
+            ref = o.ax(x)
+            scale = np.abs(o.coefs()).sum(axis=0) * np.abs(x).max()
+            assert np.max(np.abs(y - ref) / scale) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["tiny", "mid"])
+def test_rhs_phi_energy(oracle_mod, solver_mod, name):
+    say(f"[gpu] rhs/phi/energy mesh={name}")
+    mesh = _mesh(name)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    br0 = random_br0(mesh, 3)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 1) as s:
+        s.set_br0(br0)
+        b = s.rhs()
+        bref = o.rhs(br0)
+        assert np.max(np.abs(b - bref)) <= 1e-14 * np.max(np.abs(bref))
+        x = np.random.default_rng(1).standard_normal(o.N)
+        phi = s.fill_phi(x)
+        pref = o.fill_phi(x, br0)
+        assert np.max(np.abs(phi - pref)) <= 1e-14 * np.max(np.abs(pref))
+        w = s.magnetic_energy()
+        wref = o.energy(pref)
+        assert np.allclose(w, wref, rtol=1e-12, atol=0)
+        br, bt, bp = s.getb()
+        rbr, rbt, rbp = o.getb(pref)
+        for g_, r_ in ((br, rbr), (bt, rbt), (bp, rbp)):
+            assert np.max(np.abs(g_ - r_)) <= 1e-13 * np.max(np.abs(r_))
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_diag_preconditioner(oracle_mod, solver_mod, name):
+    say(f"[gpu] diag prec mesh={name}")
+    mesh = _mesh(name)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 1) as s:
+        x = np.random.default_rng(2).standard_normal(o.N)
+        assert np.max(np.abs(s.prec_inv(x) - o.prec_apply(x, 1))) <= 1e-6 * np.max(
+            np.abs(o.prec_apply(x, 1)))
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_ilu_pivots_and_apply(oracle_mod, solver_mod, name):
+    say(f"[gpu] ilu mesh={name}")
+    mesh = _mesh(name)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+        dinv = s.ilu_dinv()
+        dref = o.dilu_dinv()
+        assert np.max(np.abs(dinv - dref) / np.abs(dref)) < 1e-6
+        x = np.random.default_rng(4).standard_normal(o.N)
+        z = s.prec_inv(x)
+        zref = o.dilu_apply(x.astype(np.float32)).astype(np.float64)
+        assert np.max(np.abs(z - zref)) <= 2e-5 * np.max(np.abs(zref))
+        # the reference's own CPU ILU0 path (Saad ilu0 + lusol)
+        zs = o.prec_apply(x, 2)
+        assert np.max(np.abs(z - zs)) <= 1e-4 * np.max(np.abs(zs))
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("ifprec,oprec", [(1, 1), (2, 3), (0, 0)])
+def test_solve_matches_oracle(oracle_mod, solver_mod, ifprec, oprec):
+    say(f"[gpu] solve ifprec={ifprec}")
+    mesh = small_mesh(21, 31, 48, drratio=2.5)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    br0 = random_br0(mesh, 9)
+    b = o.rhs(br0)
+    ref = o.cgsolve(b, ifprec=oprec, epscg=1e-9)
+    with solver_mod.Pot3dSolver(mesh, -1.0, ifprec) as s:
+        s.set_br0(br0)
+        x = np.zeros(o.N)
+        r = b.copy()
+        ierr, ncg, epsn = s.solve(x, r, 1e-9)
+        assert ierr == 0
+        assert abs(ncg - ref["ncg"]) <= max(2, ref["ncg"] // 50)
+        assert epsn < 1e-9
+        assert _rel(x, ref["x"]) < 1e-8
+        # residual returned in rhs, as cgsolve's r
+        assert np.max(np.abs(r - (b - o.ax(x)))) < 1e-8 * np.max(np.abs(b))
+        phi = s.fill_phi()
+        w = s.magnetic_energy()
+        wref = o.energy(o.fill_phi(ref["x"], br0))
+        assert np.allclose(w, wref, rtol=1e-7, atol=0)
+        st = s.stats()
+        assert st["status"] == 1 and st["true_relres"] < 1e-6
+
+
+def test_two_norm_stop_and_mixed_precision(oracle_mod, solver_mod):
+    say("[gpu] mixed precision")
+    from pot3d_amd import _lib
+    mesh = small_mesh(25, 37, 60, drratio=3.0)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    br0 = random_br0(mesh, 12)
+    b = o.rhs(br0)
+    ref = o.cgsolve(b, ifprec=1, epscg=1e-12)
+    wref = o.energy(o.fill_phi(ref["x"], br0))
+    for ifprec in (1, 2):
+        with solver_mod.Pot3dSolver(mesh, -1.0, ifprec) as s:
+            s.set_br0(br0)
+            s.set_option(_lib.OPT_STOP, 1)
+            x = np.zeros(o.N)
+            assert s.solve(x, None, 1e-9)[0] == 0
+            st = s.stats()
+            assert st["true_relres"] <= 1.2e-9
+            x64 = x.copy()
+            s.set_option(_lib.OPT_MIXED, 1)
+            x = np.zeros(o.N)
+            assert s.solve(x, None, 1e-9)[0] == 0
+            st = s.stats()
+            say(f"  ifprec={ifprec} mixed: outer={st['n_outer']} ncg={st['ncg']} "
+                f"relres={st['true_relres']:.3e}")
+            assert st["true_relres"] <= 1e-9 and st["n_outer"] >= 1
+            assert _rel(x, ref["x"]) < 1e-8 and _rel(x64, ref["x"]) < 1e-8
+            s.fill_phi()
+            assert np.allclose(s.magnetic_energy(), wref, rtol=1e-7, atol=0)
+
+
+def test_failure_paths(oracle_mod, solver_mod):
+    say("[gpu] failure paths")
+    mesh = small_mesh()
+    with solver_mod.Pot3dSolver(mesh, -1.0, 1) as s:
+        s.set_br0(np.zeros((mesh.nt, mesh.np)))
+        x = np.ones(s.N)
+        ierr, ncg, epsn = s.solve(x, None, 1e-9)
+        assert ierr == 0 and ncg == 0 and np.all(x == 0)  # zero rhs (:4302-4311)
+        br0 = random_br0(mesh, 1)
+        s.set_br0(br0)
+        x = np.zeros(s.N)
+        ierr, ncg, _ = s.solve(x, None, 1e-14, ncgmax=3)
+        assert ierr == 1 and ncg == 3
+        bad = br0.copy()
+        bad[3, 4] = np.nan
+        s.set_br0(bad)
+        from pot3d_amd._lib import Pot3dError, P3D_ENONFINITE
+        with pytest.raises(Pot3dError) as e:
+            s.solve(np.zeros(s.N), None, 1e-9)
+        assert e.value.code == P3D_ENONFINITE
+
+
+def test_legacy_lu_abi(oracle_mod, solver_mod):
+    """load_external_lu_solver / external_lu_solver / unload with host
+    arrays (the -DHIPSPARSE convention without OpenMP mapping): LU factors
+    written back over the CSR values, and the trisolve, against the
+    oracle's Saad ilu0 + lusol (src/pot3d.F90:4717-4813, 5227-5302)."""
+    say("[gpu] legacy ABI")
+    import ctypes as C
+    from pot3d_amd import _lib
+    L = _lib.load()
+    mesh = small_mesh(17, 14, 26, drratio=2.0)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    IA, JA, Av, dptr = o.csr()
+    _, _, LUref, _, _ = o.ilu0()
+    lu = Av.copy()
+    L.load_external_lu_solver(lu.ctypes.data, IA.ctypes.data, JA.ctypes.data, o.N, len(Av))
+    nnz = IA[-1] - 1
+    assert np.max(np.abs(lu[:nnz] - LUref[:nnz]) / np.abs(LUref[:nnz])) < 1e-4
+    x = np.random.default_rng(5).standard_normal(o.N)
+    ref = o.lusol(x.astype(np.float32)).astype(np.float64)
+    y = x.copy()
+    L.external_lu_solver(y.ctypes.data)
+    assert np.max(np.abs(y - ref)) <= 1e-4 * np.max(np.abs(ref))
+    L.unload_external_lu_solver()
