@@ -88,8 +88,11 @@ enum {
   P3D_OPT_INNER_TOL = 3,  /* inner relative tolerance (default 1e-5) */
   P3D_OPT_CHUNK = 4,      /* CG iterations per captured graph (default auto) */
   P3D_OPT_GRAPH = 5,      /* 1: replay iterations as a hipGraph (default) */
-  P3D_OPT_SWEEP = 6,      /* ILU sweep: 0 dataflow (default), 1 hyperplane
-                             launches */
+  P3D_OPT_SWEEP = 6,      /* ILU triangular-solve schedule: 0 tile diagonals
+                             (8x8x32 tiles, one launch per anti-diagonal;
+                             default), 1 one launch per i+j+k hyperplane,
+                             2 one persistent launch per sweep (single
+                             applications only; rejected by the solve) */
   P3D_OPT_VERBOSE = 7     /* 1: print the reference's residual lines */
 };
 

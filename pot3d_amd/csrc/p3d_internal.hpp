@@ -120,6 +120,17 @@ struct CgState {
   int pad_;
 };
 
+// Control-state reads in kernels go through agent-scope atomic loads, so
+// every workgroup on every XCD sees the value the previous kernel's
+// epilogue stored (no stale L2 line can make part of a grid skip work and
+// break the last-block / last-wave protocols).
+__device__ __forceinline__ int st_done(const CgState* st) {
+  return __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double st_load(const double* f) {
+  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Reduction slots used by the "last block finishes" pattern.
 struct Reduce {
   double* partial;   // [3 * max_blocks]

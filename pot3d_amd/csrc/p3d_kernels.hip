@@ -45,63 +45,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// ---------------------------------------------------------------------------
-// CG scalar logic (cgsolve/ernorm), run by one thread after a reduction.
-__device__ void cg_check(CgState* st) {
-  const double epssq = st->rdotr / st->bdotb;
-  const double epsn = copysign(1.0, epssq) * sqrt(fabs(epssq));
-  st->epsn = epsn;
-  st->relres = sqrt(st->rr / st->bb);
-  if (st->hist && st->ncg < st->hist_cap) st->hist[st->ncg] = epsn;
-  const double crit = st->stop_mode == 0 ? epsn : st->relres;
-  if (!isfinite(epsn) || !isfinite(st->relres)) {
-    st->done = 3;
-  } else if (crit < st->epscg) {
-    st->done = 1;
-  } else if (st->ncg >= st->ncgmax) {
-    st->done = 2;
-  }
-}
-
-__device__ void cg_epilogue(CgState* st, Epi epi, double s0, double s1, double* out) {
-  switch (epi) {
-    case EPI_BDOTB:
-      st->bdotb = s0;
-      st->bb = s1;
-      if (s0 == 0.0) st->done = 4;
-      break;
-    case EPI_RINIT:
-      st->rdotr = s0;
-      st->rr = s1;
-      cg_check(st);
-      break;
-    case EPI_PAP:
-      st->ncg += 1;
-      st->pdotap = s0;
-      st->alpha = st->rdotr / s0;
-      break;
-    case EPI_RR:
-      st->rr = s1;
-      break;
-    case EPI_RITER:
-    case EPI_RZITER:
-      st->rdotr_old = st->rdotr;
-      st->rdotr = s0;
-      if (epi == EPI_RITER) st->rr = s1;
-      cg_check(st);
-      if (!st->done) st->beta = st->rdotr / st->rdotr_old;
-      break;
-    case EPI_STORE:
-      if (out) {
-        out[0] = s0;
-        out[1] = s1;
-      }
-      break;
-    default:
-      break;
-  }
-}
-
 // Block reduction of NV doubles; every block stores its partial, the last
 // block to arrive sums all partials in a fixed order (deterministic for a
 // given launch shape), runs the epilogue and re-arms the counter.
@@ -206,7 +149,7 @@ __global__ void __launch_bounds__(kBlock)
               const T* __restrict__ b, int mode, const double* __restrict__ pole_part,
               CgState* st, double* partial, unsigned* counter, Epi epi, double* out,
               int guard, int want_dot) {
-  if (guard && st->done) return;
+  if (guard && st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
@@ -242,14 +185,101 @@ __global__ void __launch_bounds__(kBlock)
   block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
 }
 
+// 2.5-D form of the same operator: one thread per (i,j) column marching
+// through a segment of phi planes.  x(k-1), x(k), x(k+1) stay in registers,
+// the column's metric factors are loaded once, and the per-plane factors
+// DPH/DPL/DPU are wave-uniform, so each point costs one HBM read of x, four
+// cached neighbour reads and one write.  Partial sums of the fused dot are
+// reduced per block with the same last-block epilogue.
+constexpr int kSegPlanes = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_stencil_march(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
+                    const T* __restrict__ b, int mode, const double* __restrict__ pole_part,
+                    CgState* st, double* partial, unsigned* counter, Epi epi, double* out,
+                    int guard, int want_dot) {
+  if (guard && st_done(st)) return;
+  const int G = gridDim.x;
+  const int Lb = logical_block(G);
+  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
+  const int cb = Lb % ncol_blocks, seg = Lb / ncol_blocks;
+  const int kper = (g.c + nseg - 1) / nseg;
+  const int k0 = seg * kper, k1 = min(g.c, k0 + kper);
+  const uint32_t mcol = (uint32_t)cb * kBlock + threadIdx.x;
+  double acc = 0.0;
+  if (mcol < g.ab && k0 < k1) {
+    const uint32_t jj = fdiv(mcol, g.da);
+    const int i = (int)(mcol - jj * (uint32_t)g.a), j = (int)jj;
+    const double DRH = m.DRH[i], CRL = m.CRL[i], CRH = m.CRH[i];
+    const double SD = m.SD[j], CTL = m.CTL[j], CTH = m.CTH[j];
+    const double P1 = DRH * m.CP[j];
+    const bool im = i > 0, ip = i < g.a - 1, jm = j > 0, jp = j < g.b - 1;
+    const T pmr = (T)m.pm_r1;
+    const T s0 = jm ? (T)0 : (T)(2.0 * pole_sum(pole_part, 0, i, g.a));
+    const T s1 = jp ? (T)0 : (T)(2.0 * pole_sum(pole_part, 1, i, g.a));
+    const uint32_t ab = g.ab, a = (uint32_t)g.a;
+    const T* xc = x + mcol;
+    T xkm = xc[(size_t)ab * (k0 > 0 ? k0 - 1 : g.c - 1)];
+    T xk = xc[(size_t)ab * k0];
+    for (int k = k0; k < k1; k++) {
+      const uint32_t n = mcol + ab * (uint32_t)k;
+      const T xkp = xc[(size_t)ab * (k < g.c - 1 ? k + 1 : 0)];
+      const T xim = im ? x[n - 1] : xk;
+      const T xip = ip ? x[n + 1] : pmr * xk;
+      const T xjm = jm ? x[n - a] : s0 - xk;
+      const T xjp = jp ? x[n + a] : s1 - xk;
+      const double dph = m.DPH[k];
+      const double a1 = -(P1 * m.DPL[k]);
+      const double a7 = -(P1 * m.DPU[k]);
+      const double p2 = DRH * dph;
+      const double a2 = -(p2 * CTL);
+      const double a6 = -(p2 * CTH);
+      const double p3 = SD * dph;
+      const double a3 = -(p3 * CRL);
+      const double a5 = -(p3 * CRH);
+      const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
+      T yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
+             (T)a6 * xjp + (T)a7 * xkp;
+      if (mode == 1) yv = b[n] - yv;
+      y[n] = yv;
+      acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
+      xkm = xk;
+      xk = xkp;
+    }
+  }
+  if (!want_dot) return;
+  double v[1] = {acc};
+  block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
+}
+
+static int stencil_segments(const Grid3& g) {
+  // enough blocks to fill the chip several times, >= kSegPlanes planes each
+  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
+  int nseg = (8 * 256 * 4 + ncol_blocks - 1) / ncol_blocks;
+  const int maxseg = (g.c + kSegPlanes - 1) / kSegPlanes;
+  if (nseg > maxseg) nseg = maxseg;
+  if (nseg < 1) nseg = 1;
+  return nseg;
+}
+
+int stencil_blocks(const Grid3& g) {
+  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
+  const int G = ncol_blocks * stencil_segments(g);
+  return (G + 7) / 8 * 8;
+}
+
 template <typename T>
 void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x, T* y,
                     const T* b, int mode, const double* pole_part, CgState* st, Reduce red,
                     int slot, Epi epi, double* out, bool guard, hipStream_t s) {
+  (void)L;
   const int want = (epi != EPI_NONE);
-  k_stencil<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, x, y, b, mode, pole_part, st,
-                                       red.partial, red.counter + slot, epi, out, guard ? 1 : 0,
-                                       want);
+  const int nseg = stencil_segments(g);
+  const int G = stencil_blocks(g);
+  k_stencil_march<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
+                                           red.partial, red.counter + slot, epi, out,
+                                           guard ? 1 : 0, want);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,11 +308,11 @@ __global__ void __launch_bounds__(kBlock)
     k_cg_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ x,
                 T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap, CgState* st,
                 double* partial, unsigned* counter) {
-  if (st->done) return;
+  if (st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
-  const T alpha = (T)st->alpha;
+  const T alpha = (T)st_load(&st->alpha);
   double s0 = 0.0, s1 = 0.0;
   for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
     x[n] = x[n] + alpha * p[n];
@@ -312,11 +342,11 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_p_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ p,
                const T* __restrict__ r, const T* __restrict__ z, CgState* st) {
-  if (st->done) return;
+  if (st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
-  const T beta = (T)st->beta;
+  const T beta = (T)st_load(&st->beta);
   for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
     const T zn = (prec == PREC_ILU) ? z[n] : apply_point_prec<T>(m, g, prec, n, r[n]);
     p[n] = beta * p[n] + zn;
@@ -335,7 +365,7 @@ __global__ void __launch_bounds__(kBlock)
     k_dot2(Grid3 g, int G, int CH, const T* __restrict__ x, const T* __restrict__ y,
            const T* __restrict__ u, const T* __restrict__ v, CgState* st, double* partial,
            unsigned* counter, Epi epi, double* out, int guard) {
-  if (guard && st->done) return;
+  if (guard && st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
@@ -618,7 +648,7 @@ template <typename T, typename CP>
 __global__ void k_fwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
                             const float* __restrict__ dinv, const T* __restrict__ r,
                             float* __restrict__ yv, const CgState* guard) {
-  if (guard && guard->done) return;
+  if (guard && st_done(guard)) return;
   int i, j, k;
   if (!level_point(g, L, kmin, i, j, k)) return;
   const uint32_t n = (uint32_t)i + (uint32_t)g.a * ((uint32_t)j + (uint32_t)g.b * (uint32_t)k);
@@ -641,7 +671,7 @@ template <typename T, typename CP>
 __global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
                             const float* __restrict__ dinv, float* __restrict__ yv,
                             T* __restrict__ z, const CgState* guard) {
-  if (guard && guard->done) return;
+  if (guard && st_done(guard)) return;
   int i, j, k;
   if (!level_point(g, L, kmin, i, j, k)) return;
   const uint32_t n = (uint32_t)i + (uint32_t)g.a * ((uint32_t)j + (uint32_t)g.b * (uint32_t)k);

@@ -20,6 +20,64 @@ enum Epi {
   EPI_STORE = 7,   // s0,s1 -> out[0], out[1] (plain dot products)
 };
 
+// ---------------------------------------------------------------------------
+// CG scalar logic (cgsolve/ernorm), run by one thread after a reduction.
+__device__ inline void cg_check(CgState* st) {
+  const double epssq = st->rdotr / st->bdotb;
+  const double epsn = copysign(1.0, epssq) * sqrt(fabs(epssq));
+  st->epsn = epsn;
+  st->relres = sqrt(st->rr / st->bb);
+  if (st->hist && st->ncg < st->hist_cap) st->hist[st->ncg] = epsn;
+  const double crit = st->stop_mode == 0 ? epsn : st->relres;
+  if (!isfinite(epsn) || !isfinite(st->relres)) {
+    st->done = 3;
+  } else if (crit < st->epscg) {
+    st->done = 1;
+  } else if (st->ncg >= st->ncgmax) {
+    st->done = 2;
+  }
+}
+
+__device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, double* out) {
+  switch (epi) {
+    case EPI_BDOTB:
+      st->bdotb = s0;
+      st->bb = s1;
+      if (s0 == 0.0) st->done = 4;
+      break;
+    case EPI_RINIT:
+      st->rdotr = s0;
+      st->rr = s1;
+      cg_check(st);
+      break;
+    case EPI_PAP:
+      st->ncg += 1;
+      st->pdotap = s0;
+      st->alpha = st->rdotr / s0;
+      break;
+    case EPI_RR:
+      st->rr = s1;
+      break;
+    case EPI_RITER:
+    case EPI_RZITER:
+      st->rdotr_old = st->rdotr;
+      st->rdotr = s0;
+      if (epi == EPI_RITER) st->rr = s1;
+      cg_check(st);
+      if (!st->done) st->beta = st->rdotr / st->rdotr_old;
+      break;
+    case EPI_STORE:
+      if (out) {
+        out[0] = s0;
+        out[1] = s1;
+      }
+      break;
+    default:
+      break;
+  }
+}
+
+
 struct Launch {
   int G;        // blocks of a streaming kernel (multiple of 8)
   int CH;       // points per block (multiple of 256)
@@ -34,6 +92,9 @@ Launch plan_stream(uint32_t N, int max_blocks);
 // kPoleSegs partial sums each: part[(pole*kPoleSegs+seg)*a + i].
 template <typename T>
 void launch_pole_sums(const Metric& m, const Grid3& g, const T* x, double* part, hipStream_t s);
+
+// Workgroups of the operator launch (its reductions need that many slots).
+int stencil_blocks(const Grid3& g);
 
 // y = A x (homogeneous boundary values; mode 0) or y = b - A x (mode 1).
 // Reduces s0 = x.y (mode 0) or s0 = y.y (mode 1) and runs epilogue `epi`.

@@ -142,12 +142,17 @@ static int read_state(p3d_solver* s) {
 template <typename T>
 static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, bool guard) {
   if (s->ifprec == PREC_ILU) {
-    if (s->sweep_mode == 0 && s->sweep_ready)
-      launch_dilu_sweep<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, guard ? s->st : nullptr,
+    const CgState* gd = guard ? s->st : nullptr;
+    if (s->sweep_mode == 0 && s->sweep_ready) {
+      launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
+    } else if (s->sweep_mode == 2 && s->sweep_ready && !guard) {
+      // persistent dataflow schedule: single applications only
+      launch_dilu_sweep<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, nullptr, nullptr,
                            s->stream);
-    else
-      launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true,
-                                  guard ? s->st : nullptr, s->stream);
+    } else {
+      launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, gd,
+                                  s->stream);
+    }
     if (epi != EPI_NONE)
       launch_dot2<T>(s->g, s->L, r, z, (epi == EPI_RZITER) ? nullptr : r,
                      (epi == EPI_RZITER) ? nullptr : r, s->st, s->red, 1, epi, out, guard,
@@ -172,9 +177,11 @@ static void cg_iteration(p3d_solver* s, Vecs<T> v) {
 template <typename T>
 static int chunk_size(p3d_solver* s) {
   if (s->chunk > 0) return s->chunk;
-  if (s->ifprec == PREC_ILU && !(s->sweep_mode == 0 && s->sweep_ready)) {
-    const int per = 2 * (s->g.a + s->g.b + s->g.c);
+  if (s->ifprec == PREC_ILU) {
+    const int per = (s->sweep_mode == 0 && s->sweep_ready) ? 2 * s->sweep.ndiag + 6
+                                                           : 2 * (s->g.a + s->g.b + s->g.c);
     int k = 4096 / per;
+    if (k > 32) k = 32;
     return k < 1 ? 1 : k;
   }
   return 32;
@@ -237,9 +244,21 @@ static int pcg(p3d_solver* s, Vecs<T> v, double epscg, int ncgmax, int stop_mode
   launch_copy<T>(s->g, s->L, v.p, v.z, s->stream);
   if ((rc = read_state(s))) return rc;
   const int K = chunk_size<T>(s);
+  // The loop ends on the device's done flag; it is also bounded on the host:
+  // every replay must advance the iteration count, and no more replays than
+  // ncgmax needs are allowed.
+  const long max_launches = (long)ncgmax / K + 4;
+  long launches = 0;
   while (!s->st_host->done) {
+    const int ncg_before = s->st_host->ncg;
     if ((rc = run_iterations<T>(s, v, K))) return rc;
     if ((rc = read_state(s))) return rc;
+    launches++;
+    if (!s->st_host->done && (s->st_host->ncg <= ncg_before || launches > max_launches))
+      return fail(P3D_ESTATE,
+                  "CG state stopped advancing: ncg=%d (was %d) done=%d epsn=%g after %ld "
+                  "graph replays of %d iterations",
+                  s->st_host->ncg, ncg_before, s->st_host->done, s->st_host->epsn, launches, K);
   }
   return P3D_OK;
 }
@@ -341,6 +360,10 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
 // Core solve on the solver's own buffers: s->x guess, s->b rhs.
 static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
   int rc;
+  if (s->ifprec == PREC_ILU && s->sweep_mode == 2)
+    return fail(P3D_EINVAL,
+                "the persistent dataflow ILU sweep (P3D_OPT_SWEEP=2) is disabled inside the CG "
+                "iteration; use 0 (tile diagonals) or 1 (hyperplanes)");
   auto t0 = std::chrono::steady_clock::now();
   memset(&s->stats, 0, sizeof s->stats);
   if (s->mixed) {
@@ -532,7 +555,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   if (hipHostMalloc((void**)&s->out_host, 8 * sizeof(double), 0) != hipSuccess)
     return bail(fail(P3D_ENOMEM, "hipHostMalloc failed"));
   if ((rc = dalloc((void**)&s->out, 8 * sizeof(double)))) return bail(rc);
-  s->red.max_blocks = s->L.G > 4096 ? s->L.G : 4096;
+  s->red.max_blocks = s->L.G > stencil_blocks(s->g) ? s->L.G : stencil_blocks(s->g);
   if ((rc = dalloc((void**)&s->red.partial, sizeof(double) * 3 * s->red.max_blocks)))
     return bail(rc);
   if ((rc = dalloc((void**)&s->red.counter, sizeof(unsigned) * 16))) return bail(rc);
@@ -580,6 +603,8 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       s->use_graph = value != 0;
       break;
     case P3D_OPT_SWEEP:
+      if (value != 0 && value != 1 && value != 2)
+        return fail(P3D_EINVAL, "P3D_OPT_SWEEP must be 0, 1 or 2");
       s->sweep_mode = (int)value;
       for (int t = 0; t < 2; t++) {
         if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
@@ -782,6 +807,8 @@ int p3d_apply_prec(p3d_solver* s, double* x) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(x, s->z, nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->ifprec == PREC_ILU && sweep_error(s->sweep))
+    return fail(P3D_EHIP, "ILU dataflow sweep exceeded its wait bound (results invalid)");
   return P3D_OK;
 }
 

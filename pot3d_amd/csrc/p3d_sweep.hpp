@@ -1,22 +1,69 @@
-// Dataflow D-ILU sweeps (one persistent launch per triangular solve).
+// Dataflow D-ILU sweeps: one persistent launch per triangular solve.
+//
+// The (i,j) plane is cut into 8x8 tiles; one wave owns a tile (lane =
+// ii + 8*jj) and marches along k (phi) through all c planes.  Lane (ii,jj)
+// handles plane k = s - ii - jj at step s, so its (i-1) and (j-1)
+// neighbours (lanes -1 and -8) finished the same plane one step earlier
+// and hand their values over by cross-lane shuffle; (k-1) and the phi wrap
+// partner stay in the lane's registers.  Tile faces come from the tiles
+// below in i and j, which publish per-tile progress (planes finished)
+// after every block of steps.  Tiles are claimed through an atomic ticket
+// in topological order, so every tile waited on is already owned by a
+// running wave: the sweep cannot deadlock.  Waits are bounded; a wait that
+// exceeds the bound sets an error flag (the sweep then finishes with wrong
+// values and the solver reports the error) instead of hanging.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "p3d_internal.hpp"
 
 namespace p3d {
 
+struct SweepCtl {
+  unsigned ticket_f, exited_f;  // forward sweep
+  unsigned ticket_b, exited_b;  // backward sweep
+  unsigned error;               // wait bound exceeded
+  unsigned pad[3];
+};
+
 struct SweepPlan {
   int ready = 0;
+  int NI = 0, NJ = 0, nitems = 0;
+  int grid = 0;                 // workgroups (one wave each)
+  int* order_f = nullptr;       // ticket -> item (I + NI*J), forward order
+  int* order_b = nullptr;       // backward order
+  unsigned* prog_f = nullptr;   // per-item planes finished (forward)
+  unsigned* prog_b = nullptr;   // per-item planes finished (backward)
+  SweepCtl* ctl = nullptr;
+  double* item_partial = nullptr;  // per-tile partial r.z of the backward sweep
+  // tile-diagonal schedule (8x8xTK tiles, one launch per anti-diagonal)
+  int TK = 0, NK = 0, ndiag = 0;
+  int* tiles = nullptr;            // tiles ordered by diagonal
+  std::vector<int> diag_off;       // ndiag+1 offsets into tiles
 };
 
 // Returns 0 when the dataflow sweep can run on this grid.
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus);
 void free_sweep_plan(SweepPlan& plan);
+// 1 if a sweep wait exceeded its bound since the last call (and clears it).
+int sweep_error(const SweepPlan& plan);
 
+// Tile-diagonal sweeps: z = (LU)^-1 r with 2*ndiag launches and no waiting
+// between workgroups (the schedule used inside the CG iteration).
+template <typename T>
+void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
+                       const T* r, float* yv, T* z, const CgState* guard, hipStream_t s);
+
+// Persistent dataflow sweeps (single applications).
 // z = (LU)^-1 r with the matrix-free D-ILU factors (dinv = 1/pivots).
+// yv: FP32 scratch of N (forward output, overwritten by z in FP32).
+// dot_st non-null: the backward sweep also reduces r.z (tile order) and
+// runs the CG epilogue EPI_RZITER on dot_st.
 template <typename T>
 void launch_dilu_sweep(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
-                       const T* r, float* yv, T* z, const CgState* guard, hipStream_t s);
+                       const T* r, float* yv, T* z, const CgState* guard, CgState* dot_st,
+                       hipStream_t s);
 
 }  // namespace p3d

@@ -108,8 +108,91 @@ def test_ilu_pivots_and_apply(oracle_mod, solver_mod, name):
         assert np.max(np.abs(z - zs)) <= 1e-4 * np.max(np.abs(zs))
 
 
+@pytest.mark.parametrize("dims", [(9, 8, 12), (13, 11, 18), (21, 31, 48), (40, 19, 30),
+                                  (33, 45, 70)])
+def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
+    """The three triangular-solve schedules (tile diagonals, one launch per
+    hyperplane, one persistent launch per sweep) do the same arithmetic per
+    point: results must be bit-identical, on every repeat (the persistent
+    sweep re-arms its tickets and progress counters itself)."""
+    say(f"[gpu] ilu dataflow vs hyperplanes dims={dims}")
+    from pot3d_amd import _lib
+    mesh = small_mesh(*dims)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+        rng = np.random.default_rng(8)
+        for rep in range(3):
+            x = rng.standard_normal(s.N)
+            s.set_option(_lib.OPT_SWEEP, 2)
+            z_df = s.prec_inv(x)
+            z_df2 = s.prec_inv(x)
+            s.set_option(_lib.OPT_SWEEP, 1)
+            z_lv = s.prec_inv(x)
+            s.set_option(_lib.OPT_SWEEP, 0)
+            z_tl = s.prec_inv(x)
+            z_tl2 = s.prec_inv(x)
+            assert np.array_equal(z_df, z_lv), (rep, np.max(np.abs(z_df - z_lv)))
+            assert np.array_equal(z_df, z_df2)
+            assert np.array_equal(z_tl, z_lv), (rep, np.max(np.abs(z_tl - z_lv)))
+            assert np.array_equal(z_tl, z_tl2)
+
+
+@pytest.mark.skip(reason="the persistent dataflow sweep (P3D_OPT_SWEEP=0) inside the CG "
+                         "iteration loop did not terminate on the MI355X (three runs); that path "
+                         "is switched off in the solver until its cause is shown, see DESIGN.md")
+def test_ilu_solve_same_with_both_sweeps(solver_mod):
+    say("[gpu] ilu solve dataflow vs hyperplanes")
+    from pot3d_amd import _lib
+    mesh = small_mesh(21, 31, 48, drratio=2.5)
+    br0 = random_br0(mesh, 9)
+    out = []
+    for mode in (0, 1):
+        with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+            s.set_br0(br0)
+            s.set_option(_lib.OPT_SWEEP, mode)
+            x = np.zeros(s.N)
+            ierr, ncg, epsn = s.solve(x, None, 1e-10, ncgmax=2000)
+            assert ierr == 0
+            out.append((ncg, x))
+    # same preconditioner bit for bit; the dataflow path reduces r.z inside
+    # the backward sweep (tile order), so the CG scalars differ in the last
+    # bits only
+    assert abs(out[0][0] - out[1][0]) <= 1
+    assert _rel(out[0][1], out[1][1]) < 1e-10
+
+
 def _rel(a, b):
     return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def test_ilu_solve_tiles_vs_hyperplanes(solver_mod):
+    """ILU0 PCG with the tile-diagonal schedule and with one launch per
+    hyperplane: identical preconditioner bits, so identical iterates."""
+    say("[gpu] ilu solve tiles vs hyperplanes")
+    from pot3d_amd import _lib
+    mesh = small_mesh(21, 31, 48, drratio=2.5)
+    br0 = random_br0(mesh, 9)
+    out = []
+    for mode in (0, 1):
+        with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+            s.set_br0(br0)
+            s.set_option(_lib.OPT_SWEEP, mode)
+            x = np.zeros(s.N)
+            ierr, ncg, epsn = s.solve(x, None, 1e-10, ncgmax=2000)
+            assert ierr == 0
+            out.append((ncg, x))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_dataflow_sweep_rejected_in_solve(solver_mod):
+    from pot3d_amd import _lib
+    mesh = small_mesh()
+    with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+        s.set_br0(random_br0(mesh, 1))
+        s.set_option(_lib.OPT_SWEEP, 2)
+        with pytest.raises(_lib.Pot3dError) as e:
+            s.solve(np.zeros(s.N), None, 1e-9, ncgmax=10)
+        assert e.value.code == _lib.P3D_EINVAL
 
 
 @pytest.mark.parametrize("ifprec,oprec", [(1, 1), (2, 3), (0, 0)])
@@ -124,7 +207,7 @@ def test_solve_matches_oracle(oracle_mod, solver_mod, ifprec, oprec):
         s.set_br0(br0)
         x = np.zeros(o.N)
         r = b.copy()
-        ierr, ncg, epsn = s.solve(x, r, 1e-9)
+        ierr, ncg, epsn = s.solve(x, r, 1e-9, ncgmax=5000)
         assert ierr == 0
         assert abs(ncg - ref["ncg"]) <= max(2, ref["ncg"] // 50)
         assert epsn < 1e-9
@@ -153,13 +236,13 @@ def test_two_norm_stop_and_mixed_precision(oracle_mod, solver_mod):
             s.set_br0(br0)
             s.set_option(_lib.OPT_STOP, 1)
             x = np.zeros(o.N)
-            assert s.solve(x, None, 1e-9)[0] == 0
+            assert s.solve(x, None, 1e-9, ncgmax=5000)[0] == 0
             st = s.stats()
             assert st["true_relres"] <= 1.2e-9
             x64 = x.copy()
             s.set_option(_lib.OPT_MIXED, 1)
             x = np.zeros(o.N)
-            assert s.solve(x, None, 1e-9)[0] == 0
+            assert s.solve(x, None, 1e-9, ncgmax=5000)[0] == 0
             st = s.stats()
             say(f"  ifprec={ifprec} mixed: outer={st['n_outer']} ncg={st['ncg']} "
                 f"relres={st['true_relres']:.3e}")
@@ -175,7 +258,7 @@ def test_failure_paths(oracle_mod, solver_mod):
     with solver_mod.Pot3dSolver(mesh, -1.0, 1) as s:
         s.set_br0(np.zeros((mesh.nt, mesh.np)))
         x = np.ones(s.N)
-        ierr, ncg, epsn = s.solve(x, None, 1e-9)
+        ierr, ncg, epsn = s.solve(x, None, 1e-9, ncgmax=100)
         assert ierr == 0 and ncg == 0 and np.all(x == 0)  # zero rhs (:4302-4311)
         br0 = random_br0(mesh, 1)
         s.set_br0(br0)
@@ -187,7 +270,7 @@ def test_failure_paths(oracle_mod, solver_mod):
         s.set_br0(bad)
         from pot3d_amd._lib import Pot3dError, P3D_ENONFINITE
         with pytest.raises(Pot3dError) as e:
-            s.solve(np.zeros(s.N), None, 1e-9)
+            s.solve(np.zeros(s.N), None, 1e-9, ncgmax=1000)
         assert e.value.code == P3D_ENONFINITE
 
 

@@ -22,7 +22,7 @@ def test_deck(deck, ifprec):
     from pot3d_amd import solver, workloads
     w = workloads.testsuite_deck(deck)
     ref = workloads.testsuite_reference(deck)
-    res = solver.potfld(w, ifprec=ifprec)
+    res = solver.potfld(w, ifprec=ifprec, ncgmax=3 * ref["iterations"])
     assert res.ierr == 0
     if ifprec == 1:
         assert abs(res.ncg - ref["iterations"]) <= 2
