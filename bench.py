@@ -91,6 +91,33 @@ def cpu_baseline(workload, gpu_iters: int, sample_iters: int, threads: int):
                        f"{gpu_iters} iterations")}
 
 
+# (name, time_kernel id, algorithmic bytes per unknown, padded-array bytes per point)
+KERNELS = [
+    ("stencil_fp64 (k_stencil_march<double>)", 0, 8, 8),
+    ("stencil_fp32 (k_stencil_march<float>)", 1, 4, 4),
+    ("ilu_apply_fp64 (k_tile_fwd+k_tile_bwd<double>)", 2, 32, 0),
+    ("ilu_apply_fp32 (k_tile_fwd+k_tile_bwd<float>)", 7, 24, 0),
+    ("cg_update_fp64 (k_cg_update<double>)", 4, 48, 0),
+    ("p_update_fp64 (k_p_update<double>)", 5, 24, 0),
+    ("copy_fp64 (k_copy<double>)", 6, 16, 0),
+]
+
+
+def measure_kernels(s, N: int, padded: int, reps: int = 20):
+    """Average HIP-event time per launch on the solver's stream, algorithmic
+    bytes per launch and the achieved bandwidth of each kernel."""
+    table = {}
+    for name, kid, per_n, per_pad in KERNELS:
+        ms = s.time_kernel(kid, reps if kid not in (2, 7) else 5)
+        nbytes = per_n * N + per_pad * padded
+        table[name] = {"ms": round(ms, 5), "bytes": nbytes,
+                       "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    # the dominant kernel of the solve: the ILU apply or the operator, by time
+    cand = {k: v for k, v in table.items() if k.startswith(("ilu_apply_fp64", "stencil_fp64"))}
+    name = max(cand, key=lambda k: cand[k]["ms"])
+    return {"table": table, "dominant": dict(name=name, **table[name])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +126,8 @@ def main():
     ap.add_argument("--grid", type=str, default="151x301x601")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only time the kernels (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -117,6 +146,11 @@ def main():
     s.set_br0(w.br0)
     s.set_option(_lib.OPT_STOP, 1)
     rhs = s.rhs()
+    if args.kernels_only:
+        kern = measure_kernels(s, N, nr * nt * np_, reps=5)
+        print(json.dumps({"workload": w.name, "unknowns": N, "kernels": kern["table"]}))
+        s.close()
+        return
     d_x = hip.malloc(8 * N)
     d_b = hip.malloc(8 * N)
     zero = np.zeros(N)
@@ -158,32 +192,12 @@ def main():
     hip.d2h(xmx, d_x)
     rel_x = float(np.max(np.abs(xmx - x64)) / np.max(np.abs(x64)))
 
-    # kernel timing (HIP events on the solver's stream)
-    reps = 20
-    ms_sten = s.time_kernel(0, reps)
-    ms_sten32 = s.time_kernel(1, reps)
-    ms_ilu = s.time_kernel(2, 5)
-    padded = nr * nt * np_
-    bytes_sten = 8 * padded + 8 * N     # matrix-free FP64: x (with halo) + y
-    bytes_sten32 = 4 * padded + 4 * N
-    bytes_ilu = 32 * N                  # r, dinv, y (fwd) + y, dinv, z (bwd)
-    gbs_sten = bytes_sten / (ms_sten * 1e-3) / 1e9
-    gbs_ilu = bytes_ilu / (ms_ilu * 1e-3) / 1e9
+    kern = measure_kernels(s, N, nr * nt * np_)
+    dom = kern["dominant"]
+    roof = {"kernel": dom["name"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_ms": dom["ms"], "bytes_per_launch": dom["bytes"]}
     per_iter64 = min(t64) / max(st64["ncg"], 1)
-    dominant = "ilu_apply" if ms_ilu > ms_sten else "stencil_fp64"
-    if dominant == "ilu_apply":
-        roof = {"kernel": "D-ILU forward+backward sweep (FP64 in/out, FP32 inside)",
-                "bound": "hbm", "achieved": round(gbs_ilu, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(gbs_ilu / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_ms": round(ms_ilu, 4), "bytes_per_launch": bytes_ilu}
-    else:
-        roof = {"kernel": "k_stencil<double>", "bound": "hbm", "achieved": round(gbs_sten, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs_sten / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_ms": round(ms_sten, 4), "bytes_per_launch": bytes_sten}
-    stencil = {"fp64_GBs": round(gbs_sten, 1), "fp64_frac": round(gbs_sten / HBM_PEAK_GBS, 4),
-               "fp64_ms": round(ms_sten, 4),
-               "fp32_GBs": round(bytes_sten32 / (ms_sten32 * 1e-3) / 1e9, 1),
-               "fp32_ms": round(ms_sten32, 4)}
     cpu = None
     if not args.no_cpu:
         try:
@@ -205,7 +219,7 @@ def main():
         "speedup_vs_fp64": round(min(t64) / best, 3),
         "mixed_vs_fp64_max_rel_dx": rel_x,
         "fp64_ms_per_iter": round(1e3 * per_iter64, 4),
-        "stencil": stencil, "ilu_apply_ms": round(ms_ilu, 4),
+        "kernels": kern["table"],
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -17,6 +17,7 @@
 
 #include "../../include/pot3d_hip.h"
 #include "p3d_kernels.hpp"
+#include "p3d_sweep.hpp"
 
 using namespace p3d;
 
@@ -39,6 +40,8 @@ struct Legacy {
   float* dinv = nullptr;
   float* yv = nullptr;
   double* xbuf = nullptr;   // staging for host x
+  SweepPlan plan;           // tile-diagonal schedule of the trisolves
+  bool tiles = false;
 };
 
 Legacy G;
@@ -134,6 +137,7 @@ void unload() {
   if (G.dinv) hipFree(G.dinv);
   if (G.yv) hipFree(G.yv);
   if (G.xbuf) hipFree(G.xbuf);
+  free_sweep_plan(G.plan);
   if (G.stream) hipStreamDestroy(G.stream);
   G = Legacy();
 }
@@ -275,6 +279,11 @@ void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M
     check(hipMemcpy(dv, AV.data(), nnz * sizeof(float), hipMemcpyHostToDevice), "write LU");
   else
     memcpy(csr_lu, AV.data(), nnz * sizeof(float));
+  int dev = 0, ncu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    ncu = prop.multiProcessorCount;
+  G.tiles = (build_sweep_plan(G.plan, G.g, ncu) == 0);
   G.loaded = true;
 }
 
@@ -287,10 +296,14 @@ void external_lu_solver(double* x) {
     check(hipMemcpyAsync(G.xbuf, x, nb, hipMemcpyHostToDevice, G.stream), "copy x");
     dx = G.xbuf;
   }
-  Metric m;
-  memset(&m, 0, sizeof m);
-  launch_dilu_solve_levels<double>(m, G.g, G.dinv, dx, G.yv, dx, &G.ca, G.wrap, nullptr,
-                                   G.stream);
+  if (G.tiles) {
+    launch_dilu_tiles_stored(G.plan, G.ca, G.g, G.dinv, dx, G.yv, dx, G.stream);
+  } else {
+    Metric m;
+    memset(&m, 0, sizeof m);
+    launch_dilu_solve_levels<double>(m, G.g, G.dinv, dx, G.yv, dx, &G.ca, G.wrap, nullptr,
+                                     G.stream);
+  }
   check(hipGetLastError(), "trisolve launch");
   if (dx == G.xbuf) check(hipMemcpyAsync(x, G.xbuf, nb, hipMemcpyDeviceToHost, G.stream), "copy x");
   check(hipStreamSynchronize(G.stream), "trisolve");

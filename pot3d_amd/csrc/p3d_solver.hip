@@ -822,7 +822,8 @@ int p3d_ilu_dinv(p3d_solver* s, float* dinv) {
 int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   if (!s || !avg_ms || reps <= 0) return fail(P3D_EINVAL, "bad argument");
   if (which == 2 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
-  if (which == 1 || which == 3) {
+  if (which == 7 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
+  if (which == 1 || which == 3 || which == 7) {
     int rc = ensure_mixed(s);
     if (rc) return rc;
   }
@@ -863,6 +864,12 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
         launch_p_update<double>(s->m, s->g, s->L, s->ifprec, s->p, s->r, s->z, s->st,
                                 s->stream);
         break;
+      case 6:  // plain copy of N doubles: the bandwidth reference
+        launch_copy<double>(s->g, s->L, s->ap, s->p, s->stream);
+        break;
+      case 7:  // FP32 ILU apply (the mixed-precision inner preconditioner)
+        if (s->ifprec == PREC_ILU) prec_and_dot<float>(s, s->r32, s->z32, EPI_NONE, nullptr, false);
+        break;
       default:
         break;
     }
@@ -878,7 +885,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   *avg_ms = (double)ms / reps;
-  if (which > 5 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
+  if (which > 7 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
   return P3D_OK;
 }
 

@@ -303,8 +303,55 @@ __global__ void __launch_bounds__(64)
 // by earlier launches, so there is no waiting between workgroups at all;
 // inside a tile the lanes march through the TK planes with the same skewed
 // schedule, shuffles and arithmetic as the dataflow sweep.
-template <typename T>
-__device__ void fwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, int k1,
+// Coefficient sources of the tile sweeps: matrix-free (metric factors,
+// the solver) or stored FP32 couplings (the legacy CSR input).
+struct TileMF {
+  Metric m;
+  double DRH, CRL, CRH, SD, CTL, CTH, P1;
+  __device__ __forceinline__ void init(int ic, int jc, uint32_t) {
+    DRH = m.DRH[ic];
+    CRL = m.CRL[ic];
+    CRH = m.CRH[ic];
+    SD = m.SD[jc];
+    CTL = m.CTL[jc];
+    CTH = m.CTH[jc];
+    P1 = DRH * m.CP[jc];
+  }
+  __device__ __forceinline__ float aw(uint32_t) const { return (float)(-(P1 * m.DPU[m.c - 1])); }
+  __device__ __forceinline__ float bw(uint32_t) const { return (float)(-(P1 * m.DPL[0])); }
+  __device__ __forceinline__ void lower(int k, uint32_t, float& ak, float& aj, float& ai) const {
+    const double dph = m.DPH[k];
+    ak = (float)(-(P1 * m.DPL[k]));
+    aj = (float)(-((DRH * dph) * CTL));
+    ai = (float)(-((SD * dph) * CRL));
+  }
+  __device__ __forceinline__ void upper(int k, uint32_t, float& bk, float& bj, float& bi) const {
+    const double dph = m.DPH[k];
+    bk = (float)(-(P1 * m.DPU[k]));
+    bj = (float)(-((DRH * dph) * CTH));
+    bi = (float)(-((SD * dph) * CRH));
+  }
+};
+
+struct TileStored {
+  CoefArrays ca;
+  __device__ __forceinline__ void init(int, int, uint32_t) {}
+  __device__ __forceinline__ float aw(uint32_t col) const { return ca.wrap_lo[col]; }
+  __device__ __forceinline__ float bw(uint32_t col) const { return ca.wrap_up[col]; }
+  __device__ __forceinline__ void lower(int, uint32_t p, float& ak, float& aj, float& ai) const {
+    ak = ca.lo_k[p];
+    aj = ca.lo_j[p];
+    ai = ca.lo_i[p];
+  }
+  __device__ __forceinline__ void upper(int, uint32_t p, float& bk, float& bj, float& bi) const {
+    bk = ca.up_k[p];
+    bj = ca.up_j[p];
+    bi = ca.up_i[p];
+  }
+};
+
+template <typename T, typename CS>
+__device__ void fwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
                          const float* __restrict__ dinv, const T* __restrict__ r,
                          float* __restrict__ yv) {
   const int lane = threadIdx.x;
@@ -313,9 +360,8 @@ __device__ void fwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, 
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float aw = (float)(-(P1 * m.DPU[g.c - 1]));
+  cs.init(ic, jc, col);
+  const float aw = (k1 == g.c) ? cs.aw(col) : 0.f;
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (act && k0 > 0) {
     const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
@@ -333,10 +379,7 @@ __device__ void fwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, 
         const uint32_t p = col + g.ab * (uint32_t)k;
         rr[u] = (float)r[p];
         dd[u] = dinv[p];
-        const double dph = m.DPH[k];
-        ak[u] = (float)(-(P1 * m.DPL[k]));
-        aj[u] = (float)(-((DRH * dph) * CTL));
-        ai[u] = (float)(-((SD * dph) * CRL));
+        cs.lower(k, p, ak[u], aj[u], ai[u]);
         if (ii == 0 && i > 0) fl[u] = dinv[p - 1] * yv[p - 1];
         if (jj == 0 && j > 0) fd[u] = dinv[p - g.a] * yv[p - g.a];
       }
@@ -362,8 +405,8 @@ __device__ void fwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, 
   }
 }
 
-template <typename T>
-__device__ void bwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, int k1,
+template <typename T, typename CS>
+__device__ void bwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
                          const float* __restrict__ dinv, float* __restrict__ yv,
                          T* __restrict__ z) {
   const int lane = threadIdx.x;
@@ -373,9 +416,8 @@ __device__ void bwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, 
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float bw = (float)(-(P1 * m.DPL[0]));
+  cs.init(ic, jc, col);
+  const float bw = (k0 == 0) ? cs.bw(col) : 0.f;
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
   if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
@@ -390,10 +432,7 @@ __device__ void bwd_tile(const Metric& m, const Grid3& g, int I, int J, int k0, 
         const uint32_t p = col + g.ab * (uint32_t)k;
         yy[u] = yv[p];
         dd[u] = dinv[p];
-        const double dph = m.DPH[k];
-        bk[u] = (float)(-(P1 * m.DPU[k]));
-        bj[u] = (float)(-((DRH * dph) * CTH));
-        bi[u] = (float)(-((SD * dph) * CRH));
+        cs.upper(k, p, bk[u], bj[u], bi[u]);
         if (ii == TI - 1 && i < g.a - 1) fr[u] = yv[p + 1];
         if (jj == TJ - 1 && j < g.b - 1) fu[u] = yv[p + g.a];
       }
@@ -426,26 +465,41 @@ struct TileArgs {
   const int* tiles;  // tiles of this diagonal: I + NI*(J + NJ*Kc)
 };
 
-template <typename T>
+template <typename T, typename CS>
 __global__ void __launch_bounds__(64)
-    k_tile_fwd(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
+    k_tile_fwd(CS cs, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
   if (guard && st_done(guard)) return;
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
   const int k0 = Kc * ta.TK, k1 = min(g.c, k0 + ta.TK);
-  fwd_tile<T>(m, g, I, J, k0, k1, dinv, r, yv);
+  fwd_tile<T, CS>(cs, g, I, J, k0, k1, dinv, r, yv);
 }
 
-template <typename T>
+template <typename T, typename CS>
 __global__ void __launch_bounds__(64)
-    k_tile_bwd(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
+    k_tile_bwd(CS cs, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
   if (guard && st_done(guard)) return;
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
   const int k0 = Kc * ta.TK, k1 = min(g.c, k0 + ta.TK);
-  bwd_tile<T>(m, g, I, J, k0, k1, dinv, yv, z);
+  bwd_tile<T, CS>(cs, g, I, J, k0, k1, dinv, yv, z);
+}
+
+template <typename T, typename CS>
+void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const float* dinv,
+                  const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
+  for (int L = 0; L < plan.ndiag; L++) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
+    k_tile_fwd<T, CS><<<cnt, 64, 0, s>>>(cs, g, ta, dinv, r, yv, guard);
+  }
+  for (int L = plan.ndiag - 1; L >= 0; L--) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
+    k_tile_bwd<T, CS><<<cnt, 64, 0, s>>>(cs, g, ta, dinv, yv, z, guard);
+  }
 }
 
 }  // namespace
@@ -550,16 +604,17 @@ void launch_dilu_sweep(const SweepPlan& plan, const Metric& m, const Grid3& g, c
 template <typename T>
 void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
                        const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
-  for (int L = 0; L < plan.ndiag; L++) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_fwd<T><<<cnt, 64, 0, s>>>(m, g, ta, dinv, r, yv, guard);
-  }
-  for (int L = plan.ndiag - 1; L >= 0; L--) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_bwd<T><<<cnt, 64, 0, s>>>(m, g, ta, dinv, yv, z, guard);
-  }
+  TileMF cs;
+  cs.m = m;
+  tiles_launch<T, TileMF>(plan, cs, g, dinv, r, yv, z, guard, s);
+}
+
+void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,
+                              const float* dinv, const double* r, float* yv, double* z,
+                              hipStream_t s) {
+  TileStored cs;
+  cs.ca = ca;
+  tiles_launch<double, TileStored>(plan, cs, g, dinv, r, yv, z, nullptr, s);
 }
 
 template void launch_dilu_tiles<double>(const SweepPlan&, const Metric&, const Grid3&,

@@ -17,7 +17,7 @@
 
 #include <vector>
 
-#include "p3d_internal.hpp"
+#include "p3d_kernels.hpp"
 
 namespace p3d {
 
@@ -55,6 +55,11 @@ int sweep_error(const SweepPlan& plan);
 template <typename T>
 void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
                        const T* r, float* yv, T* z, const CgState* guard, hipStream_t s);
+
+// Tile-diagonal sweeps with the stored FP32 couplings of a legacy CSR input.
+void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,
+                              const float* dinv, const double* r, float* yv, double* z,
+                              hipStream_t s);
 
 // Persistent dataflow sweeps (single applications).
 // z = (LU)^-1 r with the matrix-free D-ILU factors (dinv = 1/pivots).
