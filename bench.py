@@ -118,6 +118,23 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20):
     return {"table": table, "dominant": dict(name=name, **table[name])}
 
 
+def _coarse_br(args_tuple):
+    """br0 of the seeded synthetic map on the coarse mesh (worker process)."""
+    mesh, seed = args_tuple
+    from pot3d_amd import inputs
+    return inputs.synthetic_br0(mesh, seed)
+
+
+def make_brs(mesh, seeds, workers: int):
+    """Boundary maps for the training set, generated on the host before the
+    GPU is touched (a fork-based pool; workers=1 runs inline)."""
+    if workers <= 1 or len(seeds) < 8:
+        return [_coarse_br((mesh, sd)) for sd in seeds]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_coarse_br, [(mesh, sd) for sd in seeds], chunksize=8)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,18 +145,37 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the kernels (for profiler runs)")
+    ap.add_argument("--mixed-mode", type=int, default=2,
+                    help="1 restarted defect correction, 2 reliable updates")
+    ap.add_argument("--ws-pairs", type=int, default=512,
+                    help="training pairs of the warm-start network (0: no network)")
+    ap.add_argument("--ws-epochs", type=int, default=3)
+    ap.add_argument("--ws-lr", type=float, default=1e-3)
+    ap.add_argument("--ws-grid", type=str, default="51x91x181")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from pot3d_amd import _lib, solver, workloads
+    from pot3d_amd import workloads
     nr, nt, np_ = (int(v) for v in args.grid.split("x"))
     t0 = time.perf_counter()
     w = workloads.pfss(nr, nt, np_, seed=0, ifprec=2, name=f"pfss_{args.grid}")
     print(f"[bench] workload {w.name} N={w.N} built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
+    ws = None
+    if args.ws_pairs > 0 and not args.kernels_only:
+        cnr, cnt, cnp = (int(v) for v in args.ws_grid.split("x"))
+        wc = workloads.pfss(cnr, cnt, cnp, seed=0, ifprec=2, name=f"pfss_{args.ws_grid}")
+        t0 = time.perf_counter()
+        seeds = list(range(1000, 1000 + args.ws_pairs))
+        brs = make_brs(wc.mesh, seeds, min(16, os.cpu_count() or 1))
+        ws = dict(wc=wc, brs=brs, t_maps=time.perf_counter() - t0)
+        print(f"[bench] {len(brs)} training maps on {args.ws_grid} in {ws['t_maps']:.1f}s",
+              file=sys.stderr, flush=True)
+
+    from pot3d_amd import _lib, solver
     hip = Hip()
     N = w.N
     s = solver.Pot3dSolver(w.mesh, w.pm_r1, 2)
@@ -155,12 +191,17 @@ def main():
     d_b = hip.malloc(8 * N)
     zero = np.zeros(N)
 
-    def run(mixed: bool):
-        s.set_option(_lib.OPT_MIXED, 1 if mixed else 0)
+    def run(mixed: int, guess=None):
+        """One timed solve; guess(d_x) writes the initial guess on the device
+        inside the timed region (None: the zero guess, copied before)."""
+        s.set_option(_lib.OPT_MIXED, mixed)
+        s.set_option(_lib.OPT_GUESS_CHECK, 1 if guess else 0)
         hip.h2d(d_x, zero)
         hip.h2d(d_b, rhs)
         hip.sync()
         t = time.perf_counter()
+        if guess:
+            guess(d_x)
         ierr, ncg, epsn = s.solve_dev(d_x, d_b, 1e-9, ncgmax=20000)
         hip.sync()
         dt = time.perf_counter() - t
@@ -168,35 +209,92 @@ def main():
         assert ierr == 0, f"solve did not converge ({st})"
         return dt, st
 
+    def timed(label, mixed, guess=None):
+        for _ in range(args.warmup):
+            run(mixed, guess)
+        ts = []
+        for _ in range(args.steps):
+            dt, st = run(mixed, guess)
+            ts.append(dt)
+            print(f"[bench] {label}: {dt:.3f}s ncg={st['ncg']} outer={st['n_outer']} "
+                  f"guess={st['guess']} relres={st['true_relres']:.2e}", file=sys.stderr,
+                  flush=True)
+        x = np.empty(N)
+        hip.d2h(x, d_x)
+        return ts, st, x
+
     # FP64 plain ILU0 PCG (drop-in baseline)
-    for _ in range(args.warmup):
-        run(False)
-    t64 = []
-    for _ in range(args.steps):
-        dt, st64 = run(False)
-        t64.append(dt)
-        print(f"[bench] fp64 ILU0 PCG: {dt:.3f}s ncg={st64['ncg']} "
-              f"relres={st64['true_relres']:.2e}", file=sys.stderr, flush=True)
-    x64 = np.empty(N)
-    hip.d2h(x64, d_x)
-    # mixed precision
-    for _ in range(args.warmup):
-        run(True)
-    tmx = []
-    for _ in range(args.steps):
-        dt, stmx = run(True)
-        tmx.append(dt)
-        print(f"[bench] mixed ILU0 PCG: {dt:.3f}s ncg={stmx['ncg']} outer={stmx['n_outer']} "
-              f"relres={stmx['true_relres']:.2e}", file=sys.stderr, flush=True)
-    xmx = np.empty(N)
-    hip.d2h(xmx, d_x)
-    rel_x = float(np.max(np.abs(xmx - x64)) / np.max(np.abs(x64)))
+    t64, st64, x64 = timed("fp64 ILU0 PCG", 0)
+    # mixed precision from the zero guess
+    tmx, stmx, xmx = timed(f"mixed(mode {args.mixed_mode}) ILU0 PCG", args.mixed_mode)
+    w64 = s.magnetic_energy(s.fill_phi(x64))
+
+    def errs(x):
+        wx = s.magnetic_energy(s.fill_phi(x))
+        return {"max_rel_dphi": float(np.max(np.abs(x - x64)) / np.max(np.abs(x64))),
+                "max_rel_denergy": float(np.max(np.abs(wx - w64) / np.abs(w64)))}
+
+    result_ws = None
+    tbest, stbest, label = tmx, stmx, "mixed"
+    if ws is not None:
+        from pot3d_amd import warmstart as W
+        wc = ws["wc"]
+        sc = solver.Pot3dSolver(wc.mesh, wc.pm_r1, 2)
+        sc.set_option(_lib.OPT_STOP, 1)
+        t0 = time.perf_counter()
+        xs, iters = W.generate_pairs(sc, ws["brs"])
+        t_pairs = time.perf_counter() - t0
+        print(f"[bench] {len(xs)} training solves ({iters} PCG iterations) in {t_pairs:.1f}s",
+              file=sys.stderr, flush=True)
+        net = W.WarmStartNet(wc.mesh, seed=0)
+        losses, t_train = W.train_network(net, ws["brs"], xs, epochs=args.ws_epochs,
+                                          lr=args.ws_lr, seed=1)
+        print(f"[bench] trained {losses.size} steps in {t_train:.1f}s: loss "
+              f"{float(np.mean(losses[:16])):.3e} -> {float(np.mean(losses[-16:])):.3e}",
+              file=sys.stderr, flush=True)
+        # the fine problem's br0 on the coarse mesh, and the coarse padded phi
+        from pot3d_amd import inputs
+        brc = inputs.synthetic_br0(wc.mesh, 0)
+        sc.set_br0(brc)
+        tab = W.prolong_tables(wc.mesh, w.mesh)
+        d_phic = hip.malloc(8 * wc.mesh.nr * wc.mesh.nt * wc.mesh.np)
+
+        def guess(dx):
+            xc = net.predict(brc)
+            phic = sc.fill_phi(xc)
+            hip.h2d(d_phic, phic)
+            W.prolong_dev(tab, d_phic, dx)
+
+        twl, stwl, xwl = timed(f"mixed(mode {args.mixed_mode}) + learned ILU0 PCG",
+                               args.mixed_mode, guess)
+        t64w, st64w, _ = timed("fp64 + learned ILU0 PCG", 0, guess)
+        # quality of the guess alone
+        hip.h2d(d_x, zero)
+        guess(d_x)
+        xg = np.empty(N)
+        hip.d2h(xg, d_x)
+        g_err = float(np.linalg.norm(xg - x64) / np.linalg.norm(x64))
+        result_ws = {
+            "coarse_grid": args.ws_grid, "pairs": len(xs), "epochs": args.ws_epochs,
+            "params": net.nparams, "t_maps_s": round(ws["t_maps"], 2),
+            "t_pair_solves_s": round(t_pairs, 2), "t_train_s": round(t_train, 2),
+            "loss_first": float(np.mean(losses[:16])), "loss_last": float(np.mean(losses[-16:])),
+            "guess_rel_err": g_err, "mixed_learned_s": round(min(twl), 4),
+            "mixed_learned_ncg": stwl["ncg"], "fp64_learned_s": round(min(t64w), 4),
+            "fp64_learned_ncg": st64w["ncg"], "guess_kept": stwl["guess"] == 1,
+            "errors_vs_fp64": errs(xwl)}
+        if min(twl) < min(tbest):
+            tbest, stbest, label = twl, stwl, "mixed+learned"
+        net.close()
+        sc.close()
+        hip.free(d_phic)
 
     kern = measure_kernels(s, N, nr * nt * np_)
     dom = kern["dominant"]
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_ms": dom["ms"], "bytes_per_launch": dom["bytes"]}
+    st_name = "stencil_fp64 (k_stencil_march<double>)"
     per_iter64 = min(t64) / max(st64["ncg"], 1)
     cpu = None
     if not args.no_cpu:
@@ -204,21 +302,24 @@ def main():
             cpu = cpu_baseline(w, st64["ncg"], args.cpu_iters, min(16, os.cpu_count() or 1))
         except Exception as e:  # reported, not fatal
             cpu = {"error": str(e)}
-    best = min(tmx)
+    best = min(tbest)
     out = {
         "metric": "PCG time-to-||r||/||b||<=1e-9 (s): mixed+learned vs FP64 plain; "
                   "stencil HBM GB/s",
         "value": round(best, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(tmx)), 2),
+        "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(tbest)), 2),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 inner", "data": "synthetic (seeded bipole+harmonic Br)",
         "config": {"workload": f"PFSS {args.grid} ILU0 PCG to ||r||2/||b||2<=1e-9",
-                   "unknowns": N, "precond": "ILU0 (D-ILU)", "epscg": 1e-9},
+                   "unknowns": N, "precond": "ILU0 (D-ILU)", "epscg": 1e-9,
+                   "value_path": label, "mixed_mode": args.mixed_mode},
         "fp64_plain_s": round(min(t64), 4), "fp64_plain_ncg": st64["ncg"],
-        "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
+        "mixed_s": round(min(tmx), 4), "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
         "speedup_vs_fp64": round(min(t64) / best, 3),
-        "mixed_vs_fp64_max_rel_dx": rel_x,
+        "mixed_errors_vs_fp64": errs(xmx),
+        "learned": result_ws,
         "fp64_ms_per_iter": round(1e3 * per_iter64, 4),
+        "stencil_fp64_GBs": kern["table"][st_name]["GBs"],
         "kernels": kern["table"],
         "roofline": roof, "cpu_baseline": cpu,
     }

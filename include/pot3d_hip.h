@@ -3,7 +3,7 @@
  * POT3D's preconditioned conjugate-gradient solve of the finite-difference
  * spherical Laplace equation.
  *
- * Two groups of entry points:
+ * Three groups of entry points:
  *
  *  1. The legacy external-LU-solver ABI, symbol for symbol the one that
  *     POT3D binds under -DHIPSPARSE (src/pot3d.F90:581-609, implemented by
@@ -16,6 +16,9 @@
  *     operator, boundary values, phi seam, dots, preconditioners and the
  *     CG iteration all run on the GPU.  The driver keeps the mesh, Br input,
  *     I/O and diagnostics and passes host arrays.
+ *
+ *  3. A learned warm start (p3d_nn_*) that proposes the initial guess of
+ *     group 2's solve; it has no counterpart in the reference.
  *
  * Conventions (group 2): every function returns 0 (P3D_OK) or an error
  * code; p3d_last_error() describes the last failure on the calling thread.
@@ -30,6 +33,7 @@
 #ifndef POT3D_HIP_H
 #define POT3D_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -83,8 +87,14 @@ enum { P3D_PREC_NONE = 0, P3D_PREC_DIAG = 1, P3D_PREC_ILU0 = 2 };
 enum {
   P3D_OPT_STOP = 1,       /* 0: epsn = sqrt(r.M^-1r / b.M^-1b) < epscg (the
                              reference's ernorm, default); 1: ||r||2/||b||2 */
-  P3D_OPT_MIXED = 2,      /* 1: FP32 inner PCG inside an FP64 defect-
-                             correction loop to ||b-Ax||2/||b||2 <= epscg */
+  P3D_OPT_MIXED = 2,      /* FP32 inner PCG inside an FP64 defect-
+                             correction loop to ||b-Ax||2/||b||2 <= epscg:
+                             1 each outer step restarts the inner PCG on
+                             the new residual (inner tolerance
+                             P3D_OPT_INNER_TOL); 2 reliable updates: the
+                             FP32 recurrence keeps its search direction and
+                             its residual is replaced by the FP64 b-Ax
+                             whenever it has fallen 1000-fold */
   P3D_OPT_INNER_TOL = 3,  /* inner relative tolerance (default 1e-5) */
   P3D_OPT_CHUNK = 4,      /* CG iterations per captured graph (default auto) */
   P3D_OPT_GRAPH = 5,      /* 1: replay iterations as a hipGraph (default) */
@@ -93,7 +103,10 @@ enum {
                              default), 1 one launch per i+j+k hyperplane,
                              2 one persistent launch per sweep (single
                              applications only; rejected by the solve) */
-  P3D_OPT_VERBOSE = 7     /* 1: print the reference's residual lines */
+  P3D_OPT_VERBOSE = 7,    /* 1: print the reference's residual lines */
+  P3D_OPT_GUESS_CHECK = 8 /* 1: before solving, replace a nonzero initial
+                             guess by zero when ||b-Ax0||2 >= ||b||2 (for
+                             guesses from the warm-start network) */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /
@@ -160,7 +173,8 @@ typedef struct {
   int ncg;              /* CG iterations (sum of inner iterations if mixed) */
   int n_outer;          /* defect-correction sweeps (mixed), else 0 */
   int status;           /* 1 converged, 2 ncgmax, 3 non-finite, 4 zero rhs */
-  int pad_;
+  int guess;            /* P3D_OPT_GUESS_CHECK: 1 guess kept, 2 replaced by
+                           zero, 0 not checked */
   double epsn;          /* final epsn (reference definition) */
   double relres;        /* final ||r||2/||b||2 of the recurrence residual */
   double true_relres;   /* ||b - A x||2/||b||2 recomputed in FP64 */
@@ -177,6 +191,55 @@ int p3d_history(p3d_solver* s, double* hist, int n);
  * 6 plain copy of N doubles (bandwidth reference), 7 ILU apply in FP32.
  * Average milliseconds per launch over reps launches. */
 int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms);
+
+/* ---- Learned warm start ----------------------------------------------------
+ * No counterpart in the reference, whose initial guess is always zero
+ * (x_cg(:)=0, src/pot3d.F90:3922-3925; cgsolve's x0 at :4244-4248).  A
+ * 4-level 3-D convolutional encoder-decoder (3x3x3 kernels, 16-128
+ * channels, BF16 weights and activations, FP32 accumulation on the matrix
+ * cores) maps br0 and the radial grid to a potential on the solver's
+ * interior points, in the packed x_cg layout.  It is trained on the device
+ * (hand-written forward, backward and Adam) on (br0, x) pairs that
+ * p3d_solve produced, and its prediction is only ever an initial guess for
+ * p3d_solve, which still runs to the same epscg.
+ * Errors are described by p3d_last_error. */
+typedef struct p3d_nn p3d_nn;
+/* Network for the interior grid of (nr, nt, np); rh(nr) is the half r mesh
+ * (for the radial input channels); seed initialises the weights. */
+int p3d_nn_create(p3d_nn** out, int nr, int nt, int np, const double* rh, unsigned seed);
+int p3d_nn_destroy(p3d_nn* n);
+size_t p3d_nn_num_params(p3d_nn* n);
+/* Input and output scales: features use br0/bscale, targets x/phiscale. */
+int p3d_nn_set_scales(p3d_nn* n, double bscale, double phiscale);
+/* FP32 master parameters (p3d_nn_num_params floats). */
+int p3d_nn_get_params(p3d_nn* n, float* params);
+int p3d_nn_set_params(p3d_nn* n, const float* params);
+/* Stores a training pair br0(nt,np), x(N) on the device. */
+int p3d_nn_add_sample(p3d_nn* n, const double* br0, const double* x);
+/* One forward/backward of the mean-square loss on (br0, x): *loss and, if
+ * grads != NULL, d loss / d params (p3d_nn_num_params floats). */
+int p3d_nn_loss_grad(p3d_nn* n, const double* br0, const double* x, double* loss,
+                     float* grads);
+/* nsteps Adam steps (lr; betas 0.9, 0.999) over the stored pairs in a
+ * seeded shuffled order; losses (nsteps floats, may be NULL) receive each
+ * step's loss. */
+int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses);
+/* x(N) predicted for br0(nt,np): host x, or device x_dev when non-NULL. */
+int p3d_nn_predict(p3d_nn* n, const double* br0, double* x, double* x_dev);
+/* Trilinear prolongation of a coarse padded field phic(nrc,ntc,npc) to the
+ * af x bf x cf interior points of a finer grid: per fine interior index, the
+ * lower coarse padded index (ir, it, ip) and the weight of the upper one
+ * (wr, wt, wp).  xf(af*bf*cf) packed as x_cg. */
+int p3d_nn_prolong(int nrc, int ntc, int npc, const double* phic, int af, int bf, int cf,
+                   const int* ir, const double* wr, const int* it, const double* wt,
+                   const int* ip, const double* wp, double* xf);
+/* Same with phic_dev and xf_dev device addresses (tables on the host). */
+int p3d_nn_prolong_dev(int nrc, int ntc, int npc, const double* phic_dev, int af, int bf, int cf,
+                       const int* ir, const double* wr, const int* it, const double* wt,
+                       const int* ip, const double* wp, double* xf_dev);
+/* One 16x16x16 BF16 matrix-core product D = A B (row-major BF16 bits in,
+ * FP32 out), for checking the operand layout the network kernels use. */
+int p3d_nn_mfma_check(const unsigned short* A, const unsigned short* B, float* D);
 
 #ifdef __cplusplus
 }

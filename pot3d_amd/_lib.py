@@ -39,15 +39,31 @@ SIGNATURES = {
     "p3d_get_stats": (C.c_int, [_vp, _vp]),
     "p3d_history": (C.c_int, [_vp, _dp, C.c_int]),
     "p3d_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "p3d_nn_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, C.c_uint]),
+    "p3d_nn_destroy": (C.c_int, [_vp]),
+    "p3d_nn_num_params": (C.c_size_t, [_vp]),
+    "p3d_nn_set_scales": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "p3d_nn_get_params": (C.c_int, [_vp, _fp]),
+    "p3d_nn_set_params": (C.c_int, [_vp, _fp]),
+    "p3d_nn_add_sample": (C.c_int, [_vp, _dp, _dp]),
+    "p3d_nn_loss_grad": (C.c_int, [_vp, _dp, _dp, _dp, _fp]),
+    "p3d_nn_train": (C.c_int, [_vp, C.c_int, C.c_double, C.c_uint, _fp]),
+    "p3d_nn_predict": (C.c_int, [_vp, _dp, _dp, _vp]),
+    "p3d_nn_prolong": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
+                                 _ip, _dp, _ip, _dp, _ip, _dp, _dp]),
+    "p3d_nn_prolong_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                     _ip, _dp, _ip, _dp, _ip, _dp, _vp]),
+    "p3d_nn_mfma_check": (C.c_int, [C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), _fp]),
 }
 
 # option keys and error codes (include/pot3d_hip.h)
 P3D_OK, P3D_EINVAL, P3D_ENOMEM, P3D_EHIP, P3D_ENOCONV, P3D_ENONFINITE, P3D_ESTATE = range(7)
-OPT_STOP, OPT_MIXED, OPT_INNER_TOL, OPT_CHUNK, OPT_GRAPH, OPT_SWEEP, OPT_VERBOSE = range(1, 8)
+OPT_STOP, OPT_MIXED, OPT_INNER_TOL, OPT_CHUNK, OPT_GRAPH, OPT_SWEEP, OPT_VERBOSE, \
+    OPT_GUESS_CHECK = range(1, 9)
 
 
 class Stats(C.Structure):
-    _fields_ = [("ncg", C.c_int), ("n_outer", C.c_int), ("status", C.c_int), ("pad_", C.c_int),
+    _fields_ = [("ncg", C.c_int), ("n_outer", C.c_int), ("status", C.c_int), ("guess", C.c_int),
                 ("epsn", C.c_double), ("relres", C.c_double), ("true_relres", C.c_double),
                 ("t_solve", C.c_double), ("t_setup", C.c_double)]
 

@@ -466,6 +466,17 @@ __global__ void k_fill_f(float* p, float v, size_t n) {
   const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_accumulate(uint32_t N, double* __restrict__ x, float* __restrict__ c,
+                             const double* __restrict__ scale) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  x[n] += *scale * (double)c[n];
+  c[n] = 0.f;
+}
+void launch_accumulate(uint32_t N, double* x, float* c, const double* scale, hipStream_t s) {
+  if (N) k_accumulate<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(N, x, c, scale);
+}
+
 void launch_fill(double* p, double v, size_t n, hipStream_t s) {
   if (n) k_fill<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, v, n);
 }

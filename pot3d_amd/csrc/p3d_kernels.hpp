@@ -18,6 +18,7 @@ enum Epi {
   EPI_RR = 5,      // s1=r.r                -> rr (ILU path; r.z comes later)
   EPI_RZITER = 6,  // s0=r.z (rr from EPI_RR) -> as EPI_RITER
   EPI_STORE = 7,   // s0,s1 -> out[0], out[1] (plain dot products)
+  EPI_REPLACE = 8, // s0=r.z, s1=r.r        -> rdotr, rr (residual replaced; p kept)
 };
 
 // ---------------------------------------------------------------------------
@@ -65,6 +66,10 @@ __device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, d
       if (epi == EPI_RITER) st->rr = s1;
       cg_check(st);
       if (!st->done) st->beta = st->rdotr / st->rdotr_old;
+      break;
+    case EPI_REPLACE:
+      st->rdotr = s0;
+      st->rr = s1;
       break;
     case EPI_STORE:
       if (out) {
@@ -139,6 +144,8 @@ void launch_cast_scale(uint32_t N, TD* dst, const TS* src, const double* scale_n
                        const double* scale_den, int op, hipStream_t s);
 
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
+// x += scale[0] * c (FP64), then c = 0 (FP32 correction accumulated).
+void launch_accumulate(uint32_t N, double* x, float* c, const double* scale, hipStream_t s);
 void launch_fill_f(float* p, float v, size_t n, hipStream_t s);
 
 // RHS from the boundary data (closed form of the boundary trick).

@@ -35,6 +35,14 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace p3d {
+// Error message shared by every entry point of the library (p3d_last_error).
+int set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace p3d
+
 #define HIPCHK(expr)                                                                \
   do {                                                                              \
     hipError_t e_ = (expr);                                                         \
@@ -89,6 +97,7 @@ struct p3d_solver {
   int use_graph = 1;
   int sweep_mode = 0;
   int verbose = 0;
+  int guess_check = 0;
   // graphs: [type][0] FP64, [1] FP32
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gK[2] = {0, 0};
@@ -211,6 +220,29 @@ static int run_iterations(p3d_solver* s, Vecs<T> v, int K) {
   return P3D_OK;
 }
 
+// Replays chunks of CG iterations until the device's done flag is set.  The
+// loop is also bounded on the host: every replay must advance the iteration
+// count, and no more replays than the remaining ncgmax needs are allowed.
+template <typename T>
+static int run_until_done(p3d_solver* s, Vecs<T> v) {
+  int rc;
+  const int K = chunk_size<T>(s);
+  const long max_launches = (long)(s->st_host->ncgmax - s->st_host->ncg) / K + 4;
+  long launches = 0;
+  while (!s->st_host->done) {
+    const int ncg_before = s->st_host->ncg;
+    if ((rc = run_iterations<T>(s, v, K))) return rc;
+    if ((rc = read_state(s))) return rc;
+    launches++;
+    if (!s->st_host->done && (s->st_host->ncg <= ncg_before || launches > max_launches))
+      return fail(P3D_ESTATE,
+                  "CG state stopped advancing: ncg=%d (was %d) done=%d epsn=%g after %ld "
+                  "graph replays of %d iterations",
+                  s->st_host->ncg, ncg_before, s->st_host->done, s->st_host->epsn, launches, K);
+  }
+  return P3D_OK;
+}
+
 // PCG on vectors v (r holds the rhs on entry, x the initial guess).
 // Returns P3D_OK with s->st_host holding the final state.
 template <typename T>
@@ -243,24 +275,7 @@ static int pcg(p3d_solver* s, Vecs<T> v, double epscg, int ncgmax, int stop_mode
   prec_and_dot<T>(s, v.r, v.z, EPI_RINIT, nullptr, false);
   launch_copy<T>(s->g, s->L, v.p, v.z, s->stream);
   if ((rc = read_state(s))) return rc;
-  const int K = chunk_size<T>(s);
-  // The loop ends on the device's done flag; it is also bounded on the host:
-  // every replay must advance the iteration count, and no more replays than
-  // ncgmax needs are allowed.
-  const long max_launches = (long)ncgmax / K + 4;
-  long launches = 0;
-  while (!s->st_host->done) {
-    const int ncg_before = s->st_host->ncg;
-    if ((rc = run_iterations<T>(s, v, K))) return rc;
-    if ((rc = read_state(s))) return rc;
-    launches++;
-    if (!s->st_host->done && (s->st_host->ncg <= ncg_before || launches > max_launches))
-      return fail(P3D_ESTATE,
-                  "CG state stopped advancing: ncg=%d (was %d) done=%d epsn=%g after %ld "
-                  "graph replays of %d iterations",
-                  s->st_host->ncg, ncg_before, s->st_host->done, s->st_host->epsn, launches, K);
-  }
-  return P3D_OK;
+  return run_until_done<T>(s, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,6 +372,95 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
   return P3D_OK;
 }
 
+// FP32 inner PCG with reliable updates (residual replacement) inside the
+// FP64 defect correction.  The Krylov recurrence runs in FP32 on vectors
+// scaled by 1/||b - A x0||; whenever its residual has fallen by the factor
+// kReliableDelta since the last replacement, the FP32 correction is added to
+// the FP64 x, and the residual is replaced by the FP64 b - A x -- keeping
+// the search direction, so the Krylov space is not restarted as in
+// solve_mixed.  Ends when ||b - A x||2/||b||2 <= epscg (true residual).
+static constexpr double kReliableDelta = 1e-3;
+
+static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
+  int rc;
+  if ((rc = ensure_mixed(s)) || (rc = ensure_hist(s, ncgmax))) return rc;
+  double rr, bb;
+  if ((rc = true_residual(s, &rr, &bb))) return rc;  // ap = b - A x
+  s->stats.status = 1;
+  if (bb == 0.0) {
+    launch_fill(s->x, 0.0, (size_t)s->g.N, s->stream);
+    s->stats.status = 4;
+    return P3D_OK;
+  }
+  double relres = sqrt(rr / bb);
+  if (!isfinite(relres)) {
+    s->stats.status = 3;
+    return fail(P3D_ENONFINITE, "non-finite residual of the initial guess");
+  }
+  s->stats.relres = s->stats.true_relres = relres;
+  if (relres <= epscg) return P3D_OK;
+  const double scale = sqrt(rr);
+  s->out_host[4] = scale;
+  s->out_host[5] = 1.0;
+  HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, 2 * sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
+  Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32};
+  launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
+  launch_fill_f(v.x, 0.f, (size_t)s->g.N, s->stream);
+  // the inner relres = ||r32|| scale / ||b|| estimates the true one
+  const double final_tgt = 0.8 * epscg;
+  CgState init;
+  memset(&init, 0, sizeof init);
+  init.epscg = fmax(kReliableDelta * relres, final_tgt);
+  init.ncgmax = ncgmax;
+  init.stop_mode = 1;
+  init.hist = s->hist;
+  init.hist_cap = s->hist_cap;
+  init.bdotb = 1.0;
+  init.bb = bb / rr;
+  *s->st_host = init;
+  HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
+  prec_and_dot<float>(s, v.r, v.z, EPI_RINIT, nullptr, false);
+  launch_copy<float>(s->g, s->L, v.p, v.z, s->stream);
+  if ((rc = read_state(s))) return rc;
+  int replacements = 0;
+  for (;;) {
+    if ((rc = run_until_done<float>(s, v))) return rc;
+    const int done = s->st_host->done;
+    if (done == 3) {
+      s->stats.status = 3;
+      return fail(P3D_ENONFINITE, "non-finite residual in the inner solve");
+    }
+    // x += scale x32 ; x32 = 0 ; ap = b - A x
+    launch_accumulate(s->g.N, s->x, v.x, s->out + 4, s->stream);
+    if ((rc = true_residual(s, &rr, &bb))) return rc;
+    relres = sqrt(rr / bb);
+    s->stats.ncg = s->st_host->ncg;
+    s->stats.n_outer = ++replacements;
+    if (!isfinite(relres)) {
+      s->stats.status = 3;
+      return fail(P3D_ENONFINITE, "non-finite residual in the defect correction");
+    }
+    if (relres <= epscg) break;
+    if (done == 2 || s->st_host->ncg >= ncgmax) {
+      s->stats.status = 2;
+      s->stats.relres = s->stats.true_relres = relres;
+      return fail(P3D_ENOCONV, "mixed-precision solve did not converge");
+    }
+    // replace the residual, keep p: r32 = ap/scale ; z = M^-1 r32 ; r.z, r.r
+    launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
+    s->st_host->epscg = fmax(kReliableDelta * relres, final_tgt);
+    s->st_host->done = 0;
+    HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice,
+                          s->stream));
+    prec_and_dot<float>(s, v.r, v.z, EPI_REPLACE, nullptr, false);
+    if ((rc = read_state(s))) return rc;
+  }
+  s->stats.relres = s->stats.true_relres = relres;
+  s->stats.epsn = s->st_host->epsn;
+  return P3D_OK;
+}
+
 // Core solve on the solver's own buffers: s->x guess, s->b rhs.
 static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
   int rc;
@@ -366,8 +470,20 @@ static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
                 "iteration; use 0 (tile diagonals) or 1 (hyperplanes)");
   auto t0 = std::chrono::steady_clock::now();
   memset(&s->stats, 0, sizeof s->stats);
+  if (s->guess_check) {
+    // a proposed guess that does not reduce the residual below ||b|| is
+    // replaced by the reference's zero guess
+    double rr, bb;
+    if ((rc = true_residual(s, &rr, &bb))) return rc;
+    if (rr < bb) {
+      s->stats.guess = 1;
+    } else {
+      launch_fill(s->x, 0.0, (size_t)s->g.N, s->stream);
+      s->stats.guess = 2;
+    }
+  }
   if (s->mixed) {
-    rc = solve_mixed(s, epscg, ncgmax);
+    rc = s->mixed == 2 ? solve_mixed_ru(s, epscg, ncgmax) : solve_mixed(s, epscg, ncgmax);
     HIPCHK(hipStreamSynchronize(s->stream));
   } else {
     HIPCHK(hipMemcpyAsync(s->r, s->b, sizeof(double) * s->g.N, hipMemcpyDeviceToDevice,
@@ -590,7 +706,9 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       s->stop_mode = (int)value;
       break;
     case P3D_OPT_MIXED:
-      s->mixed = value != 0;
+      if (value != 0 && value != 1 && value != 2)
+        return fail(P3D_EINVAL, "P3D_OPT_MIXED must be 0, 1 or 2");
+      s->mixed = (int)value;
       break;
     case P3D_OPT_INNER_TOL:
       if (!(value > 0 && value < 1)) return fail(P3D_EINVAL, "inner tolerance must be in (0,1)");
@@ -613,6 +731,9 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       break;
     case P3D_OPT_VERBOSE:
       s->verbose = value != 0;
+      break;
+    case P3D_OPT_GUESS_CHECK:
+      s->guess_check = value != 0;
       break;
     default:
       return fail(P3D_EINVAL, "unknown option %d", key);

@@ -61,7 +61,8 @@ void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const
                               const float* dinv, const double* r, float* yv, double* z,
                               hipStream_t s);
 
-// Persistent dataflow sweeps (single applications).
+// Persistent dataflow sweeps (single applications only: the solve rejects
+// this schedule).
 // z = (LU)^-1 r with the matrix-free D-ILU factors (dinv = 1/pivots).
 // yv: FP32 scratch of N (forward output, overwritten by z in FP32).
 // dot_st non-null: the backward sweep also reduces r.z (tile order) and

@@ -447,17 +447,23 @@ def synthetic_br_map(seed: int, n_bipoles: int = 8, ntheta: int = 181, nphi: int
             ang = np.arccos(np.clip(np.tensordot(c, xyz, axes=1), -1.0, 1.0))
             br += fac * amp * np.exp(-0.5 * (ang / sig) ** 2)
     # real spherical-harmonic background (unnormalised associated Legendre
-    # via scipy, then normalised numerically on the map)
+    # via scipy, each term normalised numerically on the map).  Y_lm is
+    # separable, P_l^m(cos th) trig(m ph), so the sum is one (theta x terms)
+    # by (terms x phi) product.
     from scipy.special import lpmv
-    bg = np.zeros_like(TH)
-    ct = np.cos(TH)
+    st_ = np.sin(th)
+    cols_t, cols_p, coef = [], [], []
     for l in range(1, lmax + 1):
         for m in range(0, l + 1):
-            plm = lpmv(m, l, ct)
+            plm = lpmv(m, l, np.cos(th))
             for trig in ((np.cos,) if m == 0 else (np.cos, np.sin)):
-                y = plm * trig(m * PH)
-                nrm = math.sqrt(float(np.mean(y * y * np.sin(TH))) + 1e-300)
-                bg += rng.normal() / l * y / nrm
+                tp = trig(m * ph)
+                mean = float(np.sum(plm * plm * st_) * np.sum(tp * tp)) / (th.size * ph.size)
+                nrm = math.sqrt(mean + 1e-300)
+                cols_t.append(plm)
+                cols_p.append(tp)
+                coef.append(rng.normal() / l / nrm)
+    bg = (np.stack(cols_t, 1) * np.asarray(coef)) @ np.stack(cols_p, 0)
     w = np.sin(TH)
     bg *= bg_rms / math.sqrt(float(np.sum(bg * bg * w) / np.sum(w)))
     br += bg

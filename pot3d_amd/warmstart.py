@@ -1,0 +1,205 @@
+"""Learned warm start for the PCG solve (libpot3d_hip.so, p3d_nn_*).
+
+The reference has no learned part: its initial guess is zero
+(src/pot3d.F90:3922-3925).  Here a BF16 3-D convolutional encoder-decoder
+maps the r0 boundary field br0 (and the radial grid) to the potential on the
+solver's interior points.  It is trained on the GPU by the library's own
+forward, backward and Adam kernels, on (br0, x) pairs that the plain PCG
+solve produces.  Its output is only ever the initial guess of the same PCG
+solve to the same tolerance; the solver replaces it by zero when it does
+not lower the residual below ||b|| (P3D_OPT_GUESS_CHECK).
+
+Networks are trained on a coarse grid and their prediction is carried to a
+finer grid by trilinear prolongation of the padded coarse potential (with
+its boundary values and seam, p3d_fill_phi) on the GPU.
+
+Every computation runs through the C ABI; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+def _fortran_br(br0, nt, np_):
+    b = np.asfortranarray(br0, dtype=np.float64)
+    assert b.shape == (nt, np_)
+    return b
+
+
+class WarmStartNet:
+    """The network for the interior grid of ``mesh`` (nr, nt, np)."""
+
+    def __init__(self, mesh, seed: int = 0):
+        L = _lib.load()
+        self.mesh = mesh
+        self.nr, self.nt, self.np = mesh.nr, mesh.nt, mesh.np
+        self.N = (self.nr - 2) * (self.nt - 2) * (self.np - 2)
+        rh = np.ascontiguousarray(mesh.rh, dtype=np.float64)
+        h = C.c_void_p()
+        check(L.p3d_nn_create(C.byref(h), self.nr, self.nt, self.np, rh.ctypes.data_as(_dp),
+                              int(seed) & 0xFFFFFFFF))
+        self._h = h
+        self._L = L
+        self.nparams = int(L.p3d_nn_num_params(h))
+        self.nsamples = 0
+        self.bscale = 1.0
+        self.phiscale = 1.0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.p3d_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scales(self, bscale: float, phiscale: float):
+        check(self._L.p3d_nn_set_scales(self._h, float(bscale), float(phiscale)))
+        self.bscale, self.phiscale = float(bscale), float(phiscale)
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.nparams, np.float32)
+        check(self._L.p3d_nn_get_params(self._h, p.ctypes.data_as(_fp)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        check(self._L.p3d_nn_set_params(self._h, p.ctypes.data_as(_fp)))
+
+    def add_sample(self, br0: np.ndarray, x: np.ndarray):
+        b = _fortran_br(br0, self.nt, self.np)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        assert xx.size == self.N
+        check(self._L.p3d_nn_add_sample(self._h, b.ctypes.data_as(_dp), xx.ctypes.data_as(_dp)))
+        self.nsamples += 1
+
+    def loss_grad(self, br0: np.ndarray, x: np.ndarray, grads: bool = True):
+        b = _fortran_br(br0, self.nt, self.np)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        loss = C.c_double(0.0)
+        g = np.empty(self.nparams, np.float32) if grads else None
+        check(self._L.p3d_nn_loss_grad(self._h, b.ctypes.data_as(_dp), xx.ctypes.data_as(_dp),
+                                       C.byref(loss), g.ctypes.data_as(_fp) if grads else None))
+        return loss.value, g
+
+    def train(self, nsteps: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(nsteps, 1), np.float32)
+        check(self._L.p3d_nn_train(self._h, int(nsteps), float(lr), int(seed) & 0xFFFFFFFF,
+                                   losses.ctypes.data_as(_fp)))
+        return losses[:nsteps]
+
+    def predict(self, br0: np.ndarray) -> np.ndarray:
+        b = _fortran_br(br0, self.nt, self.np)
+        x = np.empty(self.N)
+        check(self._L.p3d_nn_predict(self._h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), None))
+        return x
+
+    def predict_dev(self, br0: np.ndarray, x_ptr: int):
+        b = _fortran_br(br0, self.nt, self.np)
+        check(self._L.p3d_nn_predict(self._h, b.ctypes.data_as(_dp), None, C.c_void_p(x_ptr)))
+
+
+# -- prolongation ------------------------------------------------------------
+def _axis_table(coarse: np.ndarray, fine: np.ndarray):
+    """Lower bracketing index into the coarse half mesh and the weight of the
+    upper point, for each fine coordinate."""
+    i0 = np.searchsorted(coarse, fine, side="right") - 1
+    i0 = np.clip(i0, 0, coarse.size - 2)
+    w = (fine - coarse[i0]) / (coarse[i0 + 1] - coarse[i0])
+    return np.ascontiguousarray(i0, np.int32), np.ascontiguousarray(w, np.float64)
+
+
+@dataclass
+class ProlongTables:
+    nrc: int
+    ntc: int
+    npc: int
+    af: int
+    bf: int
+    cf: int
+    arrays: list = field(default_factory=list)  # ir, wr, it, wt, ip, wp
+
+    def args(self):
+        ir, wr, it, wt, ip, wp = self.arrays
+        return (ir.ctypes.data_as(_ip), wr.ctypes.data_as(_dp), it.ctypes.data_as(_ip),
+                wt.ctypes.data_as(_dp), ip.ctypes.data_as(_ip), wp.ctypes.data_as(_dp))
+
+
+def prolong_tables(cmesh, fmesh) -> ProlongTables:
+    """Trilinear tables from the padded half mesh (rh, th, ph) of ``cmesh``
+    to the interior half-mesh points of ``fmesh``; both meshes span the same
+    domain, so every fine interior point lies inside the coarse padded box."""
+    ir, wr = _axis_table(np.asarray(cmesh.rh), np.asarray(fmesh.rh)[1:-1])
+    it, wt = _axis_table(np.asarray(cmesh.th), np.asarray(fmesh.th)[1:-1])
+    ip, wp = _axis_table(np.asarray(cmesh.ph), np.asarray(fmesh.ph)[1:-1])
+    return ProlongTables(cmesh.nr, cmesh.nt, cmesh.np, fmesh.nr - 2, fmesh.nt - 2,
+                         fmesh.np - 2, [ir, wr, it, wt, ip, wp])
+
+
+def prolong(tab: ProlongTables, phic: np.ndarray) -> np.ndarray:
+    """Fine packed x from the coarse padded phi (host arrays)."""
+    phic = np.ascontiguousarray(phic, dtype=np.float64)
+    assert phic.size == tab.nrc * tab.ntc * tab.npc
+    xf = np.empty(tab.af * tab.bf * tab.cf)
+    check(_lib.load().p3d_nn_prolong(tab.nrc, tab.ntc, tab.npc, phic.ctypes.data_as(_dp),
+                                     tab.af, tab.bf, tab.cf, *tab.args(),
+                                     xf.ctypes.data_as(_dp)))
+    return xf
+
+
+def prolong_dev(tab: ProlongTables, phic_ptr: int, xf_ptr: int):
+    """Same on device addresses."""
+    check(_lib.load().p3d_nn_prolong_dev(tab.nrc, tab.ntc, tab.npc, C.c_void_p(phic_ptr),
+                                         tab.af, tab.bf, tab.cf, *tab.args(),
+                                         C.c_void_p(xf_ptr)))
+
+
+# -- training data -------------------------------------------------------------
+def fit_scales(brs, xs):
+    """Input and output scales from training pairs: the rms of br0 and of x."""
+    bs = float(np.sqrt(np.mean([np.mean(np.square(b)) for b in brs])))
+    ps = float(np.sqrt(np.mean([np.mean(np.square(x)) for x in xs])))
+    return (bs if bs > 0 else 1.0), (ps if ps > 0 else 1.0)
+
+
+def generate_pairs(solver, brs, epscg: float = 1e-9, ncgmax: int = 100000):
+    """Solves the plain PCG for each br0 (solver: a Pot3dSolver); returns the
+    solutions and total iterations."""
+    xs, iters = [], 0
+    for b in brs:
+        solver.set_br0(b)
+        x = np.zeros(solver.N)
+        ierr, ncg, _ = solver.solve(x, None, epscg, ncgmax)
+        if ierr != 0:
+            raise RuntimeError(f"training solve did not converge ({ncg} iterations)")
+        xs.append(x)
+        iters += ncg
+    return xs, iters
+
+
+def train_network(net: WarmStartNet, brs, xs, epochs: int = 4, lr: float = 1e-3,
+                  seed: int = 0, set_scales: bool = True):
+    """Stores the pairs on the device and trains epochs x len(brs) Adam
+    steps; returns the per-step losses and the training time."""
+    if set_scales:
+        net.set_scales(*fit_scales(brs, xs))
+    for b, x in zip(brs, xs):
+        net.add_sample(b, x)
+    t0 = time.perf_counter()
+    losses = net.train(epochs * len(brs), lr, seed)
+    return losses, time.perf_counter() - t0
