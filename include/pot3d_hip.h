@@ -209,7 +209,10 @@ typedef struct p3d_nn p3d_nn;
 int p3d_nn_create(p3d_nn** out, int nr, int nt, int np, const double* rh, unsigned seed);
 int p3d_nn_destroy(p3d_nn* n);
 size_t p3d_nn_num_params(p3d_nn* n);
-/* Input and output scales: features use br0/bscale, targets x/phiscale. */
+/* Input and output scales: features use br0/bscale, targets x/phiscale.
+ * bscale = 0 selects per-sample normalisation: each br0 is divided by its
+ * own rms s over the interior points and its x by s*phiscale (the solution
+ * is linear in br0). */
 int p3d_nn_set_scales(p3d_nn* n, double bscale, double phiscale);
 /* FP32 master parameters (p3d_nn_num_params floats). */
 int p3d_nn_get_params(p3d_nn* n, float* params);

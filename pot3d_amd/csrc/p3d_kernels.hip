@@ -222,30 +222,54 @@ __global__ void __launch_bounds__(kBlock)
     const T* xc = x + mcol;
     T xkm = xc[(size_t)ab * (k0 > 0 ? k0 - 1 : g.c - 1)];
     T xk = xc[(size_t)ab * k0];
-    for (int k = k0; k < k1; k++) {
-      const uint32_t n = mcol + ab * (uint32_t)k;
-      const T xkp = xc[(size_t)ab * (k < g.c - 1 ? k + 1 : 0)];
-      const T xim = im ? x[n - 1] : xk;
-      const T xip = ip ? x[n + 1] : pmr * xk;
-      const T xjm = jm ? x[n - a] : s0 - xk;
-      const T xjp = jp ? x[n + a] : s1 - xk;
-      const double dph = m.DPH[k];
-      const double a1 = -(P1 * m.DPL[k]);
-      const double a7 = -(P1 * m.DPU[k]);
-      const double p2 = DRH * dph;
-      const double a2 = -(p2 * CTL);
-      const double a6 = -(p2 * CTH);
-      const double p3 = SD * dph;
-      const double a3 = -(p3 * CRL);
-      const double a5 = -(p3 * CRH);
-      const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
-      T yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
-             (T)a6 * xjp + (T)a7 * xkp;
-      if (mode == 1) yv = b[n] - yv;
-      y[n] = yv;
-      acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
-      xkm = xk;
-      xk = xkp;
+    // blocks of U planes: every load of the block is issued before the
+    // arithmetic, so each thread keeps U HBM reads in flight
+    constexpr int U = sizeof(T) == 8 ? 4 : 8;
+    for (int kb = k0; kb < k1; kb += U) {
+      T xn[U], xw[U], xe[U], xs[U], xnn[U], bv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = kb + u;
+        xn[u] = xw[u] = xe[u] = xs[u] = xnn[u] = bv[u] = (T)0;
+        if (k < k1) {
+          const uint32_t n = mcol + ab * (uint32_t)k;
+          xn[u] = xc[(size_t)ab * (k < g.c - 1 ? k + 1 : 0)];
+          if (im) xw[u] = x[n - 1];
+          if (ip) xe[u] = x[n + 1];
+          if (jm) xs[u] = x[n - a];
+          if (jp) xnn[u] = x[n + a];
+          if (mode == 1) bv[u] = b[n];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = kb + u;
+        if (k < k1) {
+          const uint32_t n = mcol + ab * (uint32_t)k;
+          const T xkp = xn[u];
+          const T xim = im ? xw[u] : xk;
+          const T xip = ip ? xe[u] : pmr * xk;
+          const T xjm = jm ? xs[u] : s0 - xk;
+          const T xjp = jp ? xnn[u] : s1 - xk;
+          const double dph = m.DPH[k];
+          const double a1 = -(P1 * m.DPL[k]);
+          const double a7 = -(P1 * m.DPU[k]);
+          const double p2 = DRH * dph;
+          const double a2 = -(p2 * CTL);
+          const double a6 = -(p2 * CTH);
+          const double p3 = SD * dph;
+          const double a3 = -(p3 * CRL);
+          const double a5 = -(p3 * CRH);
+          const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
+          T yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
+                 (T)a6 * xjp + (T)a7 * xkp;
+          if (mode == 1) yv = bv[u] - yv;
+          y[n] = yv;
+          acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
+          xkm = xk;
+          xk = xkp;
+        }
+      }
     }
   }
   if (!want_dot) return;

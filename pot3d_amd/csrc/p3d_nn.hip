@@ -63,6 +63,7 @@ struct p3d_nn {
   double* br_dev = nullptr;  // one br0 (nt x np)
   // training set (device)
   std::vector<double*> ds_br;
+  std::vector<double> ds_bs;  // per-sample input scale
   std::vector<float*> ds_t;
   float* tgt = nullptr;  // current target (FP32, /phiscale)
   std::vector<void*> allocs;
@@ -150,9 +151,24 @@ static void backward(p3d_nn* n) {
            s);
 }
 
-static void make_features(p3d_nn* n, const double* br_dev) {
-  features(n->D[0], 16, n->nt, br_dev, n->decay, n->rpos, (float)(1.0 / n->bscale), n->x0,
-           n->stream);
+static void make_features(p3d_nn* n, const double* br_dev, double bs) {
+  features(n->D[0], 16, n->nt, br_dev, n->decay, n->rpos, (float)(1.0 / bs), n->x0, n->stream);
+}
+
+// Input scale of one br0(nt,np) (host): the fixed bscale, or with bscale = 0
+// the rms of br0 over the interior points (the solution is linear in br0,
+// so every sample is presented to the network at unit amplitude).
+static double sample_scale(const p3d_nn* n, const double* br0) {
+  if (n->bscale > 0) return n->bscale;
+  const int nt = n->nt, np = n->c + 2;
+  double s = 0.0;
+  for (int k = 1; k < np - 1; k++)
+    for (int j = 1; j < nt - 1; j++) s += br0[j + (size_t)nt * k] * br0[j + (size_t)nt * k];
+  s = sqrt(s / ((double)(nt - 2) * (np - 2)));
+  return s > 0 ? s : 1.0;
+}
+static double target_scale(const p3d_nn* n, double bs) {
+  return n->bscale > 0 ? n->phiscale : bs * n->phiscale;
 }
 
 extern "C" {
@@ -282,7 +298,8 @@ int p3d_nn_create(p3d_nn** out, int nr, int nt, int np, const double* rh, unsign
 size_t p3d_nn_num_params(p3d_nn* n) { return n ? n->nparam : 0; }
 
 int p3d_nn_set_scales(p3d_nn* n, double bscale, double phiscale) {
-  if (!n || !(bscale > 0) || !(phiscale > 0)) return nfail(P3D_EINVAL, "scales must be > 0");
+  if (!n || !(bscale >= 0) || !(phiscale > 0))
+    return nfail(P3D_EINVAL, "need bscale >= 0 and phiscale > 0");
   n->bscale = bscale;
   n->phiscale = phiscale;
   return P3D_OK;
@@ -316,12 +333,13 @@ int p3d_nn_add_sample(p3d_nn* n, const double* br0, const double* x) {
     return nfail(P3D_ENOMEM, "training set allocation failed");
   }
   std::vector<float> t(V);
-  const double inv = 1.0 / n->phiscale;
-  for (size_t v = 0; v < V; v++) t[v] = (float)(x[v] * inv);
+  const double bs = sample_scale(n, br0), ts = target_scale(n, bs);
+  for (size_t v = 0; v < V; v++) t[v] = (float)(x[v] / ts);
   hipMemcpy(db, br0, nb * 8, hipMemcpyHostToDevice);
   hipMemcpy(dt, t.data(), V * 4, hipMemcpyHostToDevice);
   n->ds_br.push_back(db);
   n->ds_t.push_back(dt);
+  n->ds_bs.push_back(bs);
   return P3D_OK;
 }
 
@@ -331,10 +349,11 @@ int p3d_nn_loss_grad(p3d_nn* n, const double* br0, const double* x, double* loss
   if (!n || !br0 || !x || !loss) return nfail(P3D_EINVAL, "null argument");
   const size_t nb = (size_t)n->nt * (n->c + 2), V = n->D[0].V();
   std::vector<float> t(V);
-  for (size_t v = 0; v < V; v++) t[v] = (float)(x[v] / n->phiscale);
+  const double bs = sample_scale(n, br0), ts = target_scale(n, bs);
+  for (size_t v = 0; v < V; v++) t[v] = (float)(x[v] / ts);
   hipMemcpyAsync(n->br_dev, br0, nb * 8, hipMemcpyHostToDevice, n->stream);
   hipMemcpyAsync(n->tgt, t.data(), V * 4, hipMemcpyHostToDevice, n->stream);
-  make_features(n, n->br_dev);
+  make_features(n, n->br_dev, bs);
   forward(n);
   mse((int)V, 16, n->out, n->tgt, n->g_out, n->lpart, n->loss, n->stream);
   backward(n);
@@ -365,7 +384,7 @@ int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses)
     }
     const int q = order.back();
     order.pop_back();
-    make_features(n, n->ds_br[q]);
+    make_features(n, n->ds_br[q], n->ds_bs[q]);
     forward(n);
     mse(V, 16, n->out, n->ds_t[q], n->g_out, n->lpart, dloss + it, n->stream);
     backward(n);
@@ -386,8 +405,9 @@ int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses)
 int p3d_nn_predict(p3d_nn* n, const double* br0, double* x, double* x_dev) {
   if (!n || !br0 || (!x && !x_dev)) return nfail(P3D_EINVAL, "null argument");
   const size_t nb = (size_t)n->nt * (n->c + 2), V = n->D[0].V();
+  const double bs = sample_scale(n, br0);
   hipMemcpyAsync(n->br_dev, br0, nb * 8, hipMemcpyHostToDevice, n->stream);
-  make_features(n, n->br_dev);
+  make_features(n, n->br_dev, bs);
   forward(n);
   double* dst = x_dev;
   double* tmp = nullptr;
@@ -395,7 +415,7 @@ int p3d_nn_predict(p3d_nn* n, const double* br0, double* x, double* x_dev) {
     if (hipMalloc((void**)&tmp, V * 8) != hipSuccess) return nfail(P3D_ENOMEM, "allocation failed");
     dst = tmp;
   }
-  head_to_x((int)V, 16, n->out, n->phiscale, dst, n->stream);
+  head_to_x((int)V, 16, n->out, target_scale(n, bs), dst, n->stream);
   if (x) hipMemcpyAsync(x, dst, V * 8, hipMemcpyDeviceToHost, n->stream);
   hipError_t e = hipStreamSynchronize(n->stream);
   if (tmp) hipFree(tmp);

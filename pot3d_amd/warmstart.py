@@ -170,11 +170,22 @@ def prolong_dev(tab: ProlongTables, phic_ptr: int, xf_ptr: int):
 
 
 # -- training data -------------------------------------------------------------
+def _rms_interior(br0):
+    b = np.asarray(br0)[1:-1, 1:-1]
+    return float(np.sqrt(np.mean(b * b)))
+
+
 def fit_scales(brs, xs):
-    """Input and output scales from training pairs: the rms of br0 and of x."""
-    bs = float(np.sqrt(np.mean([np.mean(np.square(b)) for b in brs])))
-    ps = float(np.sqrt(np.mean([np.mean(np.square(x)) for x in xs])))
-    return (bs if bs > 0 else 1.0), (ps if ps > 0 else 1.0)
+    """Per-sample normalisation (bscale = 0): each br0 is scaled by its own
+    interior rms s; the output scale is the mean ratio rms(x) / s over the
+    training pairs, so targets have about unit rms."""
+    ratios = []
+    for b, x in zip(brs, xs):
+        s = _rms_interior(b)
+        if s > 0:
+            ratios.append(float(np.sqrt(np.mean(np.square(x)))) / s)
+    ratio = float(np.mean(ratios)) if ratios else 1.0
+    return 0.0, (ratio if ratio > 0 else 1.0)
 
 
 def generate_pairs(solver, brs, epscg: float = 1e-9, ncgmax: int = 100000):
@@ -195,11 +206,17 @@ def generate_pairs(solver, brs, epscg: float = 1e-9, ncgmax: int = 100000):
 def train_network(net: WarmStartNet, brs, xs, epochs: int = 4, lr: float = 1e-3,
                   seed: int = 0, set_scales: bool = True):
     """Stores the pairs on the device and trains epochs x len(brs) Adam
-    steps; returns the per-step losses and the training time."""
+    steps, the learning rate stepping down to lr/4 and lr/16 for the last
+    40% and 15% of the steps; returns the per-step losses and the training
+    time."""
     if set_scales:
         net.set_scales(*fit_scales(brs, xs))
     for b, x in zip(brs, xs):
         net.add_sample(b, x)
+    total = epochs * len(brs)
+    n1 = int(0.6 * total)
+    n2 = int(0.25 * total)
+    phases = [(n1, lr), (n2, lr / 4), (total - n1 - n2, lr / 16)]
     t0 = time.perf_counter()
-    losses = net.train(epochs * len(brs), lr, seed)
-    return losses, time.perf_counter() - t0
+    losses = [net.train(n, a, seed + q) for q, (n, a) in enumerate(phases) if n > 0]
+    return np.concatenate(losses), time.perf_counter() - t0

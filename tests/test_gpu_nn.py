@@ -1,12 +1,21 @@
-"""GPU tests of the warm-start network kernels (BF16 matrix cores).
+"""GPU tests of the warm-start network (BF16 matrix cores) against the CPU
+oracle (oracle/nn_oracle.py), and of what the trained network gains.
 
-The operand layout of the 16x16x16 BF16 MFMA that every convolution kernel
-relies on is checked first against a host matmul of the same BF16 values.
+* the operand layout of the 16x16x16 BF16 MFMA every convolution uses,
+  against a host matmul of the same BF16 values;
+* forward pass, loss and parameter gradient against the oracle's bf16 mode
+  (the oracle's own gradient is pinned by finite differences on the CPU);
+* trilinear prolongation: exact on multilinear fields;
+* training lowers the loss; the trained network's guess cuts PCG
+  iterations on held-out boundary maps, with the same converged answer;
+  a bad guess is replaced by zero (P3D_OPT_GUESS_CHECK).
 """
 import sys
 
 import numpy as np
 import pytest
+
+from conftest import random_br0, small_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -42,3 +51,130 @@ def test_mfma_layout():
                                    D.ctypes.data_as(C.POINTER(C.c_float))))
     ref = bf16_value(A).astype(np.float64) @ bf16_value(B).astype(np.float64)
     assert np.max(np.abs(D - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def _oracle_params(n, seed=0):
+    import nn_oracle as N
+    p = N.init_params(seed)
+    rng = np.random.default_rng(seed + 1)
+    offs, _ = N.param_offsets()
+    for (cin, cout, _), (wo, bo) in zip(N.SPEC, offs):
+        p[bo:bo + cout] = 0.05 * rng.standard_normal(cout)
+    assert p.size == n
+    return p.astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(12, 14, 20), (9, 11, 13)])
+def test_forward_and_gradient_match_oracle(dims):
+    import nn_oracle as N
+
+    from pot3d_amd.warmstart import WarmStartNet
+    nr, nt, np_ = dims
+    say(f"[gpu] nn forward/backward vs oracle {dims}")
+    mesh = small_mesh(nr=nr, nt=nt, np_=np_)
+    net = WarmStartNet(mesh, seed=1)
+    p = _oracle_params(net.nparams)
+    net.set_params(p)
+    bscale, phiscale = 2.0, 0.5
+    net.set_scales(bscale, phiscale)
+    br0 = random_br0(mesh, seed=4)
+    a, b, c = nr - 2, nt - 2, np_ - 2
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(a * b * c) * 0.3
+    x0 = N.features(br0, np.asarray(mesh.rh), bscale, mode="bf16")
+    tgt = np.asarray(x / phiscale, np.float32).astype(np.float64).reshape(c, b, a)
+    L_o, g_o, out_o = N.loss_and_grad(p.astype(np.float64), x0, tgt, mode="bf16")
+    # forward
+    xp = net.predict(br0)
+    ref = phiscale * out_o[..., 0].ravel()
+    err = np.max(np.abs(xp - ref)) / np.max(np.abs(ref))
+    say(f"    forward max rel err {err:.2e}")
+    assert err < 2e-2
+    assert np.linalg.norm(xp - ref) <= 5e-3 * np.linalg.norm(ref)
+    # loss and gradient
+    L_g, g_g = net.loss_grad(br0, x)
+    say(f"    loss gpu {L_g:.6e} oracle {L_o:.6e}")
+    assert abs(L_g - L_o) <= 5e-3 * L_o
+    offs, _ = N.param_offsets()
+    for i, ((cin, cout, _), (wo, bo)) in enumerate(zip(N.SPEC, offs)):
+        gw, ow = g_g[wo:bo], g_o[wo:bo]
+        gb, ob = g_g[bo:bo + cout], g_o[bo:bo + cout]
+        ew = np.linalg.norm(gw - ow) / np.linalg.norm(ow)
+        eb = np.linalg.norm(gb - ob) / max(np.linalg.norm(ob), 1e-30)
+        say(f"    conv {i} ({cin}->{cout}): weight grad rel err {ew:.2e}, bias {eb:.2e}")
+        assert ew < 3e-2 and eb < 3e-2
+
+
+def test_prolongation_exact_for_multilinear():
+    from pot3d_amd import warmstart as W
+    from pot3d_amd.inputs import global_mesh
+    say("[gpu] trilinear prolongation")
+    cm = global_mesh(9, 10, 14, 1.0, 2.5, rfrac=(0.0, 1.0), drratio=(2.0,))
+    fm = global_mesh(17, 19, 27, 1.0, 2.5, rfrac=(0.0, 1.0), drratio=(2.0,))
+
+    def f(r, t, p):
+        return 1.0 + 2.0 * r - 0.5 * t + 0.25 * p + 0.3 * r * t - 0.1 * t * p + 0.05 * r * t * p
+
+    R, T, P = np.meshgrid(cm.rh, cm.th, cm.ph, indexing="ij")
+    phic = f(R, T, P).ravel(order="F")
+    tab = W.prolong_tables(cm, fm)
+    xf = W.prolong(tab, phic)
+    R, T, P = np.meshgrid(fm.rh[1:-1], fm.th[1:-1], fm.ph[1:-1], indexing="ij")
+    ref = f(R, T, P).ravel(order="F")
+    assert np.max(np.abs(xf - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def _pairs(wl_fn, seeds, solver):
+    from pot3d_amd import warmstart as W
+    brs = [wl_fn(s).br0 for s in seeds]
+    xs, iters = W.generate_pairs(solver, brs)
+    return brs, xs, iters
+
+
+def test_training_lowers_loss_and_warm_start_gains():
+    from pot3d_amd import _lib
+    from pot3d_amd import warmstart as W
+    from pot3d_amd import workloads
+    from pot3d_amd.solver import Pot3dSolver
+    say("[gpu] train on PCG pairs, warm start on held-out maps")
+
+    def wl(seed):
+        return workloads.pfss(21, 37, 72, seed, ifprec=2)
+
+    w0 = wl(0)
+    s = Pot3dSolver(w0.mesh, w0.pm_r1, 2)
+    s.set_option(_lib.OPT_STOP, 1.0)
+    brs, xs, iters = _pairs(wl, range(100, 148), s)
+    say(f"    48 training pairs, {iters} PCG iterations")
+    net = W.WarmStartNet(w0.mesh, seed=0)
+    losses, t = W.train_network(net, brs, xs, epochs=24, lr=2e-3, seed=1)
+    first, last = float(np.mean(losses[:20])), float(np.mean(losses[-20:]))
+    say(f"    loss {first:.3e} -> {last:.3e} in {t:.1f} s ({losses.size} steps)")
+    assert last < 0.2 * first
+    # held-out maps: iterations from zero vs from the network's guess
+    s.set_option(_lib.OPT_GUESS_CHECK, 1.0)
+    zero_it, warm_it = [], []
+    for seed in (7, 8, 9, 10):
+        b = wl(seed).br0
+        s.set_br0(b)
+        xz = np.zeros(s.N)
+        assert s.solve(xz, None, 1e-9)[0] == 0
+        zero_it.append(s.stats()["ncg"])
+        xw = net.predict(b)
+        assert s.solve(xw, None, 1e-9)[0] == 0
+        st = s.stats()
+        warm_it.append(st["ncg"])
+        say(f"    seed {seed}: zero {zero_it[-1]} warm {warm_it[-1]} (guess {st['guess']})")
+        say(f"      guess rel err {np.linalg.norm(net.predict(b) - xz) / np.linalg.norm(xz):.3e}")
+        assert st["guess"] == 1
+        assert st["true_relres"] <= 1.5e-9
+        assert np.linalg.norm(xw - xz) <= 1e-7 * np.linalg.norm(xz)
+    assert all(w <= z for w, z in zip(warm_it, zero_it))
+    assert np.mean(warm_it) < 0.95 * np.mean(zero_it)
+    # a guess worse than zero is replaced by zero
+    xb = 1e3 * np.random.default_rng(0).standard_normal(s.N)
+    assert s.solve(xb, None, 1e-9)[0] == 0
+    st = s.stats()
+    assert st["guess"] == 2 and st["ncg"] == zero_it[-1]
+    net.close()
+    s.close()
