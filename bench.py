@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=3)
     ap.add_argument("--ws-lr", type=float, default=1e-3)
     ap.add_argument("--ws-grid", type=str, default="51x91x181")
+    ap.add_argument("--guess-check", type=int, default=2,
+                    help="1 residual rule, 2 A-norm rule for accepting the learned guess")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -195,7 +197,7 @@ def main():
         """One timed solve; guess(d_x) writes the initial guess on the device
         inside the timed region (None: the zero guess, copied before)."""
         s.set_option(_lib.OPT_MIXED, mixed)
-        s.set_option(_lib.OPT_GUESS_CHECK, 1 if guess else 0)
+        s.set_option(_lib.OPT_GUESS_CHECK, args.guess_check if guess else 0)
         hip.h2d(d_x, zero)
         hip.h2d(d_b, rhs)
         hip.sync()
@@ -274,12 +276,17 @@ def main():
         xg = np.empty(N)
         hip.d2h(xg, d_x)
         g_err = float(np.linalg.norm(xg - x64) / np.linalg.norm(x64))
+        g_res = float(np.linalg.norm(rhs - s.ax(xg)) / np.linalg.norm(rhs))
+        # A-norm error of the guess relative to the zero guess's
+        e = x64 - xg
+        g_anorm = float(np.sqrt(max(e @ s.ax(e), 0.0) / max(x64 @ s.ax(x64), 1e-300)))
         result_ws = {
             "coarse_grid": args.ws_grid, "pairs": len(xs), "epochs": args.ws_epochs,
             "params": net.nparams, "t_maps_s": round(ws["t_maps"], 2),
             "t_pair_solves_s": round(t_pairs, 2), "t_train_s": round(t_train, 2),
             "loss_first": float(np.mean(losses[:16])), "loss_last": float(np.mean(losses[-16:])),
-            "guess_rel_err": g_err, "mixed_learned_s": round(min(twl), 4),
+            "guess_rel_err": g_err, "guess_relres": g_res, "guess_rel_anorm_err": g_anorm,
+            "guess_check_mode": args.guess_check, "mixed_learned_s": round(min(twl), 4),
             "mixed_learned_ncg": stwl["ncg"], "fp64_learned_s": round(min(t64w), 4),
             "fp64_learned_ncg": st64w["ncg"], "guess_kept": stwl["guess"] == 1,
             "errors_vs_fp64": errs(xwl)}

@@ -104,9 +104,11 @@ enum {
                              2 one persistent launch per sweep (single
                              applications only; rejected by the solve) */
   P3D_OPT_VERBOSE = 7,    /* 1: print the reference's residual lines */
-  P3D_OPT_GUESS_CHECK = 8 /* 1: before solving, replace a nonzero initial
-                             guess by zero when ||b-Ax0||2 >= ||b||2 (for
-                             guesses from the warm-start network) */
+  P3D_OPT_GUESS_CHECK = 8 /* before solving, replace an initial guess that
+                             is worse than zero by zero (for guesses from
+                             the warm-start network): 1 when ||b-Ax0||2 >=
+                             ||b||2; 2 when its A-norm error is not smaller,
+                             i.e. x0.Ax0 - 2 b.x0 >= 0 */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

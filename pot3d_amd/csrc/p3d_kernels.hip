@@ -260,9 +260,20 @@ __global__ void __launch_bounds__(kBlock)
           const double p3 = SD * dph;
           const double a3 = -(p3 * CRL);
           const double a5 = -(p3 * CRH);
-          const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
-          T yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
+          T yv;
+          if constexpr (sizeof(T) == 8) {
+            // the reference's delsq form (:5689-5760)
+            const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
+            yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
                  (T)a6 * xjp + (T)a7 * xkp;
+          } else {
+            // FP32: the same operator written with neighbour differences
+            // (a4 = -sum of the others).  For smooth x the two forms differ
+            // by cancellation: a4 xk + sum a_nb x_nb loses about
+            // eps32*cond(A) relative accuracy, the differences do not.
+            yv = (T)a1 * (xkm - xk) + (T)a2 * (xjm - xk) + (T)a3 * (xim - xk) +
+                 (T)a5 * (xip - xk) + (T)a6 * (xjp - xk) + (T)a7 * (xkp - xk);
+          }
           if (mode == 1) yv = bv[u] - yv;
           y[n] = yv;
           acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;

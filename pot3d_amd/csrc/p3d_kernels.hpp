@@ -18,7 +18,7 @@ enum Epi {
   EPI_RR = 5,      // s1=r.r                -> rr (ILU path; r.z comes later)
   EPI_RZITER = 6,  // s0=r.z (rr from EPI_RR) -> as EPI_RITER
   EPI_STORE = 7,   // s0,s1 -> out[0], out[1] (plain dot products)
-  EPI_REPLACE = 8, // s0=r.z, s1=r.r        -> rdotr, rr (residual replaced; p kept)
+  EPI_REPLACE = 8, // s0=r.z, s1=r.r        -> rdotr, rr, beta (residual replaced)
 };
 
 // ---------------------------------------------------------------------------
@@ -68,8 +68,11 @@ __device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, d
       if (!st->done) st->beta = st->rdotr / st->rdotr_old;
       break;
     case EPI_REPLACE:
+      // the recurrence stopped with rdotr_old = r.z of its last direction;
+      // the replaced residual continues the same Krylov recurrence
       st->rdotr = s0;
       st->rr = s1;
+      st->beta = s0 / st->rdotr_old;
       break;
     case EPI_STORE:
       if (out) {

@@ -376,9 +376,10 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
 // FP64 defect correction.  The Krylov recurrence runs in FP32 on vectors
 // scaled by 1/||b - A x0||; whenever its residual has fallen by the factor
 // kReliableDelta since the last replacement, the FP32 correction is added to
-// the FP64 x, and the residual is replaced by the FP64 b - A x -- keeping
-// the search direction, so the Krylov space is not restarted as in
-// solve_mixed.  Ends when ||b - A x||2/||b||2 <= epscg (true residual).
+// the FP64 x, and the residual is replaced by the FP64 b - A x; the
+// recurrence then continues (beta from the replaced residual, p = z + beta
+// p), so the Krylov space is not restarted as in solve_mixed.  Ends when
+// ||b - A x||2/||b||2 <= epscg (true residual).
 static constexpr double kReliableDelta = 1e-3;
 
 static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
@@ -447,13 +448,15 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
       s->stats.relres = s->stats.true_relres = relres;
       return fail(P3D_ENOCONV, "mixed-precision solve did not converge");
     }
-    // replace the residual, keep p: r32 = ap/scale ; z = M^-1 r32 ; r.z, r.r
+    // replace the residual and continue the recurrence: r32 = ap/scale ;
+    // z = M^-1 r32 ; beta = r.z / (r.z of the last direction) ; p = z + beta p
     launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
     s->st_host->epscg = fmax(kReliableDelta * relres, final_tgt);
     s->st_host->done = 0;
     HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice,
                           s->stream));
     prec_and_dot<float>(s, v.r, v.z, EPI_REPLACE, nullptr, false);
+    launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream);
     if ((rc = read_state(s))) return rc;
   }
   s->stats.relres = s->stats.true_relres = relres;
@@ -471,11 +474,27 @@ static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
   auto t0 = std::chrono::steady_clock::now();
   memset(&s->stats, 0, sizeof s->stats);
   if (s->guess_check) {
-    // a proposed guess that does not reduce the residual below ||b|| is
-    // replaced by the reference's zero guess
-    double rr, bb;
-    if ((rc = true_residual(s, &rr, &bb))) return rc;
-    if (rr < bb) {
+    // a proposed guess that is worse than the reference's zero guess is
+    // replaced by zero: mode 1 compares ||b - A x0|| with ||b||; mode 2
+    // compares the A-norm errors, ||x* - x0||_A < ||x*||_A <=> x0.A x0 -
+    // 2 b.x0 < 0 (the norm CG minimises)
+    bool keep;
+    if (s->guess_check == 2) {
+      launch_pole_sums<double>(s->m, s->g, s->x, s->pole_part, s->stream);
+      launch_stencil<double>(s->m, s->g, s->L, s->x, s->ap, nullptr, 0, s->pole_part, nullptr,
+                             s->red, 3, EPI_STORE, s->out, false, s->stream);
+      launch_dot2<double>(s->g, s->L, s->b, s->x, nullptr, nullptr, nullptr, s->red, 4,
+                          EPI_STORE, s->out + 2, false, s->stream);
+      HIPCHK(hipMemcpyAsync(s->out_host, s->out, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      keep = s->out_host[0] - 2.0 * s->out_host[2] < 0.0;
+    } else {
+      double rr, bb;
+      if ((rc = true_residual(s, &rr, &bb))) return rc;
+      keep = rr < bb;
+    }
+    if (keep) {
       s->stats.guess = 1;
     } else {
       launch_fill(s->x, 0.0, (size_t)s->g.N, s->stream);
@@ -733,7 +752,9 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       s->verbose = value != 0;
       break;
     case P3D_OPT_GUESS_CHECK:
-      s->guess_check = value != 0;
+      if (value != 0 && value != 1 && value != 2)
+        return fail(P3D_EINVAL, "P3D_OPT_GUESS_CHECK must be 0, 1 or 2");
+      s->guess_check = (int)value;
       break;
     default:
       return fail(P3D_EINVAL, "unknown option %d", key);

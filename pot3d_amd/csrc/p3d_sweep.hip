@@ -505,8 +505,12 @@ struct TileLds {
   double dph[kTileK], dpx[kTileK];  // DPH(k), and DPL(k) (forward) or DPU(k) (backward)
 };
 
+// Four waves stage and store the tile (8 planes each, one round of loads in
+// flight per wave); wave 0 marches.
+constexpr int kStageWaves = kTileK / 8;
+
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kStageWaves)
     k_tile_fwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                    const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
   if (guard && st_done(guard)) return;
@@ -514,22 +518,22 @@ __global__ void __launch_bounds__(64)
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
   const int k0 = Kc * kTileK, k1 = min(g.c, k0 + kTileK), nk = k1 - k0;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  // stage the tile
-  if (lane < nk) {
-    sh.dph[lane] = m.DPH[k0 + lane];
-    sh.dpx[lane] = m.DPL[k0 + lane];
+  // stage the tile: wave w loads planes 8w .. 8w+7
+  if ((int)threadIdx.x < nk) {
+    sh.dph[threadIdx.x] = m.DPH[k0 + threadIdx.x];
+    sh.dpx[threadIdx.x] = m.DPL[k0 + threadIdx.x];
   }
-  for (int q0 = 0; q0 < nk; q0 += 8) {
+  {
     float rv[8], dv[8], fiv[8], fjv[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u;
+      const int q = 8 * wave + u;
       rv[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
       if (act && q < nk) {
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
@@ -541,7 +545,7 @@ __global__ void __launch_bounds__(64)
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u;
+      const int q = 8 * wave + u;
       if (q < nk) {
         sh.v[q][lane] = rv[u];
         sh.d[q][lane] = dv[u];
@@ -550,62 +554,70 @@ __global__ void __launch_bounds__(64)
       }
     }
   }
-  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float aw = (k1 == g.c) ? (float)(-(P1 * m.DPU[m.c - 1])) : 0.f;
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (act && k0 > 0) {
-    const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
-    t_own = dinv[q] * yv[q];
-  }
-  if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
-  __syncthreads();
-  // march
-  const int S = nk + SKEW;
-  for (int s0 = 0; s0 < S; s0 += 8) {
-    float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = s0 + u - ii - jj;
-      rr[u] = dd[u] = fl[u] = fd[u] = ak[u] = aj[u] = ai[u] = 0.f;
-      if (act && q >= 0 && q < nk) {
-        rr[u] = sh.v[q][lane];
-        dd[u] = sh.d[q][lane];
-        const double dph = sh.dph[q];
-        ak[u] = (float)(-(P1 * sh.dpx[q]));
-        aj[u] = (float)(-((DRH * dph) * CTL));
-        ai[u] = (float)(-((SD * dph) * CRL));
-        fl[u] = sh.fi[q][jj];
-        fd[u] = sh.fj[q][ii];
-      }
+  if (wave == 0) {
+    if (act && k0 > 0) {
+      const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
+      t_own = dinv[q] * yv[q];
     }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = s0 + u - ii - jj;
-      const int k = k0 + q;
-      const float from_i = __shfl_up(t_pub, 1, 64);
-      const float from_j = __shfl_up(t_pub, TI, 64);
-      if (act && q >= 0 && q < nk) {
-        float y = rr[u];
-        if (k == g.c - 1) y = y - aw * t_w;
-        if (k > 0) y = y - ak[u] * t_own;
-        if (j > 0) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-        if (i > 0) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
-        sh.v[q][lane] = y;
-        const float tt = dd[u] * y;
-        if (k == 0) t_w = tt;
-        t_own = tt;
-        t_pub = tt;
-      }
-    }
+    if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
   }
   __syncthreads();
-  if (act)
-    for (int q = 0; q < nk; q++) yv[col + g.ab * (uint32_t)(k0 + q)] = sh.v[q][lane];
+  if (wave == 0) {
+    const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
+    const double P1 = DRH * m.CP[jc];
+    const float aw = (k1 == g.c) ? (float)(-(P1 * m.DPU[m.c - 1])) : 0.f;
+    const int S = nk + SKEW;
+    for (int s0 = 0; s0 < S; s0 += 8) {
+      float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int q = s0 + u - ii - jj;
+        rr[u] = dd[u] = fl[u] = fd[u] = ak[u] = aj[u] = ai[u] = 0.f;
+        if (act && q >= 0 && q < nk) {
+          rr[u] = sh.v[q][lane];
+          dd[u] = sh.d[q][lane];
+          const double dph = sh.dph[q];
+          ak[u] = (float)(-(P1 * sh.dpx[q]));
+          aj[u] = (float)(-((DRH * dph) * CTL));
+          ai[u] = (float)(-((SD * dph) * CRL));
+          fl[u] = sh.fi[q][jj];
+          fd[u] = sh.fj[q][ii];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int q = s0 + u - ii - jj;
+        const int k = k0 + q;
+        const float from_i = __shfl_up(t_pub, 1, 64);
+        const float from_j = __shfl_up(t_pub, TI, 64);
+        if (act && q >= 0 && q < nk) {
+          float y = rr[u];
+          if (k == g.c - 1) y = y - aw * t_w;
+          if (k > 0) y = y - ak[u] * t_own;
+          if (j > 0) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
+          if (i > 0) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+          sh.v[q][lane] = y;
+          const float tt = dd[u] * y;
+          if (k == 0) t_w = tt;
+          t_own = tt;
+          t_pub = tt;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = 8 * wave + u;
+      if (q < nk) yv[col + g.ab * (uint32_t)(k0 + q)] = sh.v[q][lane];
+    }
+  }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kStageWaves)
     k_tile_bwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                    float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
   if (guard && st_done(guard)) return;
@@ -613,22 +625,22 @@ __global__ void __launch_bounds__(64)
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
   const int k0 = Kc * kTileK, k1 = min(g.c, k0 + kTileK), nk = k1 - k0;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
   const int rii = TI - 1 - ii, rjj = TJ - 1 - jj;
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  if (lane < nk) {
-    sh.dph[lane] = m.DPH[k0 + lane];
-    sh.dpx[lane] = m.DPU[k0 + lane];
+  if ((int)threadIdx.x < nk) {
+    sh.dph[threadIdx.x] = m.DPH[k0 + threadIdx.x];
+    sh.dpx[threadIdx.x] = m.DPU[k0 + threadIdx.x];
   }
-  for (int q0 = 0; q0 < nk; q0 += 8) {
+  {
     float yv8[8], dv[8], fiv[8], fjv[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u;
+      const int q = 8 * wave + u;
       yv8[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
       if (act && q < nk) {
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
@@ -640,7 +652,7 @@ __global__ void __launch_bounds__(64)
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u;
+      const int q = 8 * wave + u;
       if (q < nk) {
         sh.v[q][lane] = yv8[u];
         sh.d[q][lane] = dv[u];
@@ -649,59 +661,68 @@ __global__ void __launch_bounds__(64)
       }
     }
   }
-  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float bw = (k0 == 0) ? (float)(-(P1 * m.DPL[0])) : 0.f;
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
-  if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  if (wave == 0) {
+    if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
+    if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  }
   __syncthreads();
-  const int S = nk + SKEW;
-  for (int s0 = 0; s0 < S; s0 += 8) {
-    float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
+  if (wave == 0) {
+    const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
+    const double P1 = DRH * m.CP[jc];
+    const float bw = (k0 == 0) ? (float)(-(P1 * m.DPL[0])) : 0.f;
+    const int S = nk + SKEW;
+    for (int s0 = 0; s0 < S; s0 += 8) {
+      float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = (nk - 1) - (s0 + u - rii - rjj);
-      yy[u] = dd[u] = fr[u] = fu[u] = bk[u] = bj[u] = bi[u] = 0.f;
-      if (act && q >= 0 && q < nk) {
-        yy[u] = sh.v[q][lane];
-        dd[u] = sh.d[q][lane];
-        const double dph = sh.dph[q];
-        bk[u] = (float)(-(P1 * sh.dpx[q]));
-        bj[u] = (float)(-((DRH * dph) * CTH));
-        bi[u] = (float)(-((SD * dph) * CRH));
-        fr[u] = sh.fi[q][jj];
-        fu[u] = sh.fj[q][ii];
+      for (int u = 0; u < 8; u++) {
+        const int q = (nk - 1) - (s0 + u - rii - rjj);
+        yy[u] = dd[u] = fr[u] = fu[u] = bk[u] = bj[u] = bi[u] = 0.f;
+        if (act && q >= 0 && q < nk) {
+          yy[u] = sh.v[q][lane];
+          dd[u] = sh.d[q][lane];
+          const double dph = sh.dph[q];
+          bk[u] = (float)(-(P1 * sh.dpx[q]));
+          bj[u] = (float)(-((DRH * dph) * CTH));
+          bi[u] = (float)(-((SD * dph) * CRH));
+          fr[u] = sh.fi[q][jj];
+          fu[u] = sh.fj[q][ii];
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = (nk - 1) - (s0 + u - rii - rjj);
-      const int k = k0 + q;
-      const float from_i = __shfl_down(t_pub, 1, 64);
-      const float from_j = __shfl_down(t_pub, TI, 64);
-      if (act && q >= 0 && q < nk) {
-        float sv = yy[u];
-        if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-        if (j < g.b - 1) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-        if (k < g.c - 1) sv = sv - bk[u] * t_own;
-        if (k == 0) sv = sv - bw * t_w;
-        const float zz = sv * dd[u];
-        sh.v[q][lane] = zz;
-        if (k == g.c - 1) t_w = zz;
-        t_own = zz;
-        t_pub = zz;
+      for (int u = 0; u < 8; u++) {
+        const int q = (nk - 1) - (s0 + u - rii - rjj);
+        const int k = k0 + q;
+        const float from_i = __shfl_down(t_pub, 1, 64);
+        const float from_j = __shfl_down(t_pub, TI, 64);
+        if (act && q >= 0 && q < nk) {
+          float sv = yy[u];
+          if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
+          if (j < g.b - 1) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+          if (k < g.c - 1) sv = sv - bk[u] * t_own;
+          if (k == 0) sv = sv - bw * t_w;
+          const float zz = sv * dd[u];
+          sh.v[q][lane] = zz;
+          if (k == g.c - 1) t_w = zz;
+          t_own = zz;
+          t_pub = zz;
+        }
       }
     }
   }
   __syncthreads();
-  if (act)
-    for (int q = 0; q < nk; q++) {
-      const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
-      const float zz = sh.v[q][lane];
-      yv[p] = zz;
-      z[p] = (T)zz;
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = 8 * wave + u;
+      if (q < nk) {
+        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
+        const float zz = sh.v[q][lane];
+        yv[p] = zz;
+        z[p] = (T)zz;
+      }
     }
+  }
 }
 
 template <typename T>
@@ -710,12 +731,12 @@ void tiles_launch_lds(const SweepPlan& plan, const Metric& m, const Grid3& g, co
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_fwd_lds<T><<<cnt, 64, 0, s>>>(m, g, ta, dinv, r, yv, guard);
+    k_tile_fwd_lds<T><<<cnt, 64 * kStageWaves, 0, s>>>(m, g, ta, dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_bwd_lds<T><<<cnt, 64, 0, s>>>(m, g, ta, dinv, yv, z, guard);
+    k_tile_bwd_lds<T><<<cnt, 64 * kStageWaves, 0, s>>>(m, g, ta, dinv, yv, z, guard);
   }
 }
 

@@ -171,10 +171,16 @@ def test_training_lowers_loss_and_warm_start_gains():
         assert np.linalg.norm(xw - xz) <= 1e-7 * np.linalg.norm(xz)
     assert all(w <= z for w, z in zip(warm_it, zero_it))
     assert np.mean(warm_it) < 0.95 * np.mean(zero_it)
-    # a guess worse than zero is replaced by zero
-    xb = 1e3 * np.random.default_rng(0).standard_normal(s.N)
-    assert s.solve(xb, None, 1e-9)[0] == 0
-    st = s.stats()
-    assert st["guess"] == 2 and st["ncg"] == zero_it[-1]
+    # a guess worse than zero is replaced by zero, under both rules; the
+    # network's guess is kept under the A-norm rule too
+    for mode in (1, 2):
+        s.set_option(_lib.OPT_GUESS_CHECK, mode)
+        xb = 1e3 * np.random.default_rng(0).standard_normal(s.N)
+        assert s.solve(xb, None, 1e-9)[0] == 0
+        st = s.stats()
+        assert st["guess"] == 2 and st["ncg"] == zero_it[-1]
+        xw = net.predict(b)
+        assert s.solve(xw, None, 1e-9)[0] == 0
+        assert s.stats()["guess"] == 1
     net.close()
     s.close()
