@@ -349,15 +349,35 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   const T alpha = (T)st_load(&st->alpha);
   double s0 = 0.0, s1 = 0.0;
-  for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
-    x[n] = x[n] + alpha * p[n];
-    const T rn = r[n] - alpha * ap[n];
-    r[n] = rn;
-    if (prec != PREC_ILU) {
-      const T z = apply_point_prec<T>(m, g, prec, n, rn);
-      s0 += (double)rn * (double)z;
+  // blocks of kUnroll points per thread, all loads issued first; the sums
+  // keep the point order of the plain loop
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
+    T xv[kUnroll], pv[kUnroll], rv[kUnroll], av[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      xv[u] = pv[u] = rv[u] = av[u] = (T)0;
+      if (n < n1) {
+        xv[u] = x[n];
+        pv[u] = p[n];
+        rv[u] = r[n];
+        av[u] = ap[n];
+      }
     }
-    s1 += (double)rn * (double)rn;
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      if (n < n1) {
+        x[n] = xv[u] + alpha * pv[u];
+        const T rn = rv[u] - alpha * av[u];
+        r[n] = rn;
+        if (prec != PREC_ILU) {
+          const T z = apply_point_prec<T>(m, g, prec, n, rn);
+          s0 += (double)rn * (double)z;
+        }
+        s1 += (double)rn * (double)rn;
+      }
+    }
   }
   double v[2] = {s0, s1};
   block_finish<2>(v, partial, counter, G, Lb, st, prec == PREC_ILU ? EPI_RR : EPI_RITER,
@@ -382,9 +402,25 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   const T beta = (T)st_load(&st->beta);
-  for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
-    const T zn = (prec == PREC_ILU) ? z[n] : apply_point_prec<T>(m, g, prec, n, r[n]);
-    p[n] = beta * p[n] + zn;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
+    T pv[kUnroll], zv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      pv[u] = zv[u] = (T)0;
+      if (n < n1) {
+        pv[u] = p[n];
+        zv[u] = (prec == PREC_ILU) ? z[n] : r[n];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      if (n < n1) {
+        const T zn = (prec == PREC_ILU) ? zv[u] : apply_point_prec<T>(m, g, prec, n, zv[u]);
+        p[n] = beta * pv[u] + zn;
+      }
+    }
   }
 }
 
@@ -405,12 +441,32 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   double s0 = 0.0, s1 = 0.0;
-  for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
-    s0 += (double)x[n] * (double)y[n];
-    if (u) s1 += (double)u[n] * (double)v[n];
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
+    T xv[kUnroll], yv[kUnroll], uv[kUnroll], vv[kUnroll];
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) {
+      const uint32_t n = nb + q * kBlock;
+      xv[q] = yv[q] = uv[q] = vv[q] = (T)0;
+      if (n < n1) {
+        xv[q] = x[n];
+        yv[q] = y[n];
+        if (u) {
+          uv[q] = u[n];
+          vv[q] = v[n];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) {
+      const uint32_t n = nb + q * kBlock;
+      if (n < n1) {
+        s0 += (double)xv[q] * (double)yv[q];
+        if (u) s1 += (double)uv[q] * (double)vv[q];
+      }
+    }
   }
-  double vv[2] = {s0, s1};
-  block_finish<2>(vv, partial, counter, G, Lb, st, epi, out);
+  double sums[2] = {s0, s1};
+  block_finish<2>(sums, partial, counter, G, Lb, st, epi, out);
 }
 
 template <typename T>
@@ -439,8 +495,8 @@ __global__ void __launch_bounds__(kBlock)
     s1 += (double)rn * (double)rn;
   }
   if (epi == EPI_NONE) return;
-  double vv[2] = {s0, s1};
-  block_finish<2>(vv, partial, counter, G, Lb, st, epi, out);
+  double sums[2] = {s0, s1};
+  block_finish<2>(sums, partial, counter, G, Lb, st, epi, out);
 }
 
 template <typename T>

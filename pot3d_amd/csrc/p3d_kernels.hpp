@@ -92,6 +92,7 @@ struct Launch {
 };
 
 constexpr int kBlock = 256;
+constexpr int kUnroll = 4;  // points per thread in flight in the streaming kernels
 constexpr int kPoleSegs = 16;
 
 Launch plan_stream(uint32_t N, int max_blocks);
