@@ -104,11 +104,13 @@ enum {
                              2 one persistent launch per sweep (single
                              applications only; rejected by the solve) */
   P3D_OPT_VERBOSE = 7,    /* 1: print the reference's residual lines */
-  P3D_OPT_GUESS_CHECK = 8 /* before solving, replace an initial guess that
+  P3D_OPT_GUESS_CHECK = 8, /* before solving, replace an initial guess that
                              is worse than zero by zero (for guesses from
                              the warm-start network): 1 when ||b-Ax0||2 >=
                              ||b||2; 2 when its A-norm error is not smaller,
                              i.e. x0.Ax0 - 2 b.x0 >= 0 */
+  P3D_OPT_TILE_K = 9      /* planes per tile of the tile-diagonal ILU
+                             schedule: 16, 32 (default) or 64 */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

@@ -751,6 +751,23 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
     case P3D_OPT_VERBOSE:
       s->verbose = value != 0;
       break;
+    case P3D_OPT_TILE_K: {
+      const int tk = (int)value;
+      if (tk != 16 && tk != 32 && tk != 64)
+        return fail(P3D_EINVAL, "P3D_OPT_TILE_K must be 16, 32 or 64");
+      if (s->ifprec != PREC_ILU || tk == s->sweep.TK) break;
+      HIPCHK(hipStreamSynchronize(s->stream));
+      int dev = 0, ncu = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, tk) == 0);
+      for (int t = 0; t < 2; t++) {
+        if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
+        s->gexec[t] = nullptr;
+      }
+      if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");
+      break;
+    }
     case P3D_OPT_GUESS_CHECK:
       if (value != 0 && value != 1 && value != 2)
         return fail(P3D_EINVAL, "P3D_OPT_GUESS_CHECK must be 0, 1 or 2");

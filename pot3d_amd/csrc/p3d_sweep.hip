@@ -29,7 +29,7 @@ constexpr int SKEW = TI + TJ - 2;  // steps between the first and last lane of a
 constexpr int U = 16;              // steps per prefetch block
 constexpr unsigned kSpinMax = 1u << 20;
 }  // namespace
-constexpr int kTileK = 32;         // planes per tile of the tile-diagonal sweep
+constexpr int kTileK = 32;         // default planes per tile of the tile-diagonal sweep
 namespace {
 
 __device__ __forceinline__ unsigned ld_u32(const unsigned* p) {
@@ -497,27 +497,26 @@ __global__ void __launch_bounds__(64)
 // writes its result back into the staged plane, and the tile is stored
 // plane by plane at the end.  Per-point arithmetic is that of fwd_tile /
 // bwd_tile, so the results are bit-identical to the other schedules.
+template <int TK>
 struct TileLds {
-  float v[kTileK][64];   // r (forward) or y (backward), overwritten by the result
-  float d[kTileK][64];   // dinv
-  float fi[kTileK][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
-  float fj[kTileK][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
-  double dph[kTileK], dpx[kTileK];  // DPH(k), and DPL(k) (forward) or DPU(k) (backward)
+  float v[TK][64];   // r (forward) or y (backward), overwritten by the result
+  float d[TK][64];   // dinv
+  float fi[TK][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
+  float fj[TK][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
+  double dph[TK], dpx[TK];  // DPH(k), and DPL(k) (forward) or DPU(k) (backward)
 };
 
-// Four waves stage and store the tile (8 planes each, one round of loads in
+// TK/8 waves stage and store the tile (8 planes each, one round of loads in
 // flight per wave); wave 0 marches.
-constexpr int kStageWaves = kTileK / 8;
-
-template <typename T>
-__global__ void __launch_bounds__(64 * kStageWaves)
+template <typename T, int TK>
+__global__ void __launch_bounds__(8 * TK)
     k_tile_fwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                    const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
   if (guard && st_done(guard)) return;
-  __shared__ TileLds sh;
+  __shared__ TileLds<TK> sh;
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
-  const int k0 = Kc * kTileK, k1 = min(g.c, k0 + kTileK), nk = k1 - k0;
+  const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
   const int i = I * TI + ii, j = J * TJ + jj;
@@ -616,15 +615,15 @@ __global__ void __launch_bounds__(64 * kStageWaves)
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64 * kStageWaves)
+template <typename T, int TK>
+__global__ void __launch_bounds__(8 * TK)
     k_tile_bwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
                    float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
   if (guard && st_done(guard)) return;
-  __shared__ TileLds sh;
+  __shared__ TileLds<TK> sh;
   const int t = ta.tiles[blockIdx.x];
   const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
-  const int k0 = Kc * kTileK, k1 = min(g.c, k0 + kTileK), nk = k1 - k0;
+  const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
   const int rii = TI - 1 - ii, rjj = TJ - 1 - jj;
@@ -725,19 +724,31 @@ __global__ void __launch_bounds__(64 * kStageWaves)
   }
 }
 
-template <typename T>
-void tiles_launch_lds(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
-                      const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
+template <typename T, int TK>
+void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
+                         const float* dinv, const T* r, float* yv, T* z, const CgState* guard,
+                         hipStream_t s) {
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_fwd_lds<T><<<cnt, 64 * kStageWaves, 0, s>>>(m, g, ta, dinv, r, yv, guard);
+    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, ta, dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_bwd_lds<T><<<cnt, 64 * kStageWaves, 0, s>>>(m, g, ta, dinv, yv, z, guard);
+    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, ta, dinv, yv, z, guard);
   }
+}
+
+template <typename T>
+void tiles_launch_lds(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
+                      const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
+  if (plan.TK == 16)
+    tiles_launch_lds_tk<T, 16>(plan, m, g, dinv, r, yv, z, guard, s);
+  else if (plan.TK == 64)
+    tiles_launch_lds_tk<T, 64>(plan, m, g, dinv, r, yv, z, guard, s);
+  else
+    tiles_launch_lds_tk<T, 32>(plan, m, g, dinv, r, yv, z, guard, s);
 }
 
 template <typename T, typename CS>
@@ -757,7 +768,7 @@ void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const flo
 
 }  // namespace
 
-int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus) {
+int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   plan.NI = (g.a + TI - 1) / TI;
@@ -797,7 +808,7 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus) {
     return 1;
   }
   // tile diagonals
-  plan.TK = kTileK;
+  plan.TK = (tile_k == 16 || tile_k == 64) ? tile_k : kTileK;
   plan.NK = (g.c + plan.TK - 1) / plan.TK;
   plan.ndiag = plan.NI + plan.NJ + plan.NK - 2;
   std::vector<int> tl;

@@ -45,7 +45,8 @@ struct SweepPlan {
 };
 
 // Returns 0 when the dataflow sweep can run on this grid.
-int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus);
+// tile_k: planes per tile of the tile-diagonal schedule (16, 32 or 64).
+int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32);
 void free_sweep_plan(SweepPlan& plan);
 // 1 if a sweep wait exceeded its bound since the last call (and clears it).
 int sweep_error(const SweepPlan& plan);

@@ -134,6 +134,12 @@ def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
             assert np.array_equal(z_df, z_df2)
             assert np.array_equal(z_tl, z_lv), (rep, np.max(np.abs(z_tl - z_lv)))
             assert np.array_equal(z_tl, z_tl2)
+            # tile depths 16 and 64 (32 is the default above)
+            for tk in (16, 64):
+                s.set_option(_lib.OPT_TILE_K, tk)
+                z_tk = s.prec_inv(x)
+                assert np.array_equal(z_tk, z_lv), (rep, tk, np.max(np.abs(z_tk - z_lv)))
+            s.set_option(_lib.OPT_TILE_K, 32)
 
 
 @pytest.mark.skip(reason="the persistent dataflow sweep (P3D_OPT_SWEEP=2) inside the CG "
