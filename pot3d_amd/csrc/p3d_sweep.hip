@@ -497,6 +497,32 @@ __global__ void __launch_bounds__(64)
 // writes its result back into the staged plane, and the tile is stored
 // plane by plane at the end.  Per-point arithmetic is that of fwd_tile /
 // bwd_tile, so the results are bit-identical to the other schedules.
+// Tiles of anti-diagonal L = I+J+Kc in the plan's order (Kc outer, J
+// inner): the block's tile is found by a short uniform loop over Kc instead
+// of a table read, so the first memory round trip of a launch already
+// carries the tile's data.
+struct DiagArgs {
+  int L, NI, NJ, NK;
+};
+
+__device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
+  const int kc_lo = max(0, da.L - (da.NI - 1) - (da.NJ - 1));
+  const int kc_hi = min(da.NK - 1, da.L);
+  I = J = Kc = 0;
+  for (int kc = kc_lo; kc <= kc_hi; kc++) {
+    const int rest = da.L - kc;
+    const int jlo = max(0, rest - (da.NI - 1)), jhi = min(da.NJ - 1, rest);
+    const int cnt = jhi - jlo + 1;
+    if (b < cnt) {
+      Kc = kc;
+      J = jlo + b;
+      I = rest - J;
+      return;
+    }
+    b -= cnt;
+  }
+}
+
 template <int TK>
 struct TileLds {
   float v[TK][64];   // r (forward) or y (backward), overwritten by the result
@@ -510,12 +536,13 @@ struct TileLds {
 // flight per wave); wave 0 marches.
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
-    k_tile_fwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
+    k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
                    const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
-  if (guard && st_done(guard)) return;
+  // the done flag is read with the tile's data and tested before any store
+  const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK> sh;
-  const int t = ta.tiles[blockIdx.x];
-  const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
+  int I, J, Kc;
+  diag_tile(da, blockIdx.x, I, J, Kc);
   const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
@@ -523,10 +550,23 @@ __global__ void __launch_bounds__(8 * TK)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  // stage the tile: wave w loads planes 8w .. 8w+7
+  // stage the tile: wave w loads planes 8w .. 8w+7; wave 0 also loads its
+  // column factors and incoming k values in the same round trip
+  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
+  const double P1 = DRH * m.CP[jc];
+  const double dpu_last = m.DPU[m.c - 1];
+  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  if (wave == 0) {
+    if (act && k0 > 0) {
+      const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
+      t_own = dinv[q] * yv[q];
+    }
+    if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+  }
+  double dph_s = 0.0, dpx_s = 0.0;
   if ((int)threadIdx.x < nk) {
-    sh.dph[threadIdx.x] = m.DPH[k0 + threadIdx.x];
-    sh.dpx[threadIdx.x] = m.DPL[k0 + threadIdx.x];
+    dph_s = m.DPH[k0 + threadIdx.x];
+    dpx_s = m.DPL[k0 + threadIdx.x];
   }
   {
     float rv[8], dv[8], fiv[8], fjv[8];
@@ -542,6 +582,7 @@ __global__ void __launch_bounds__(8 * TK)
         if (jj == 0 && j > 0) fjv[u] = dinv[p - g.a] * yv[p - g.a];
       }
     }
+    if (done) return;  // uniform over the block
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
@@ -553,19 +594,13 @@ __global__ void __launch_bounds__(8 * TK)
       }
     }
   }
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (wave == 0) {
-    if (act && k0 > 0) {
-      const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
-      t_own = dinv[q] * yv[q];
-    }
-    if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+  if ((int)threadIdx.x < nk) {
+    sh.dph[threadIdx.x] = dph_s;
+    sh.dpx[threadIdx.x] = dpx_s;
   }
   __syncthreads();
   if (wave == 0) {
-    const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-    const double P1 = DRH * m.CP[jc];
-    const float aw = (k1 == g.c) ? (float)(-(P1 * m.DPU[m.c - 1])) : 0.f;
+    const float aw = (k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
@@ -617,12 +652,12 @@ __global__ void __launch_bounds__(8 * TK)
 
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
-    k_tile_bwd_lds(Metric m, Grid3 g, TileArgs ta, const float* __restrict__ dinv,
+    k_tile_bwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
                    float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
-  if (guard && st_done(guard)) return;
+  const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK> sh;
-  const int t = ta.tiles[blockIdx.x];
-  const int I = t % ta.NI, J = (t / ta.NI) % ta.NJ, Kc = t / (ta.NI * ta.NJ);
+  int I, J, Kc;
+  diag_tile(da, blockIdx.x, I, J, Kc);
   const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ii = lane & (TI - 1), jj = lane / TI;
@@ -631,9 +666,18 @@ __global__ void __launch_bounds__(8 * TK)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
+  const double P1 = DRH * m.CP[jc];
+  const double dpl_first = m.DPL[0];
+  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  if (wave == 0) {
+    if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
+    if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  }
+  double dph_s = 0.0, dpx_s = 0.0;
   if ((int)threadIdx.x < nk) {
-    sh.dph[threadIdx.x] = m.DPH[k0 + threadIdx.x];
-    sh.dpx[threadIdx.x] = m.DPU[k0 + threadIdx.x];
+    dph_s = m.DPH[k0 + threadIdx.x];
+    dpx_s = m.DPU[k0 + threadIdx.x];
   }
   {
     float yv8[8], dv[8], fiv[8], fjv[8];
@@ -649,6 +693,7 @@ __global__ void __launch_bounds__(8 * TK)
         if (jj == TJ - 1 && j < g.b - 1) fjv[u] = yv[p + g.a];
       }
     }
+    if (done) return;  // uniform over the block
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
@@ -660,16 +705,13 @@ __global__ void __launch_bounds__(8 * TK)
       }
     }
   }
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (wave == 0) {
-    if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
-    if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  if ((int)threadIdx.x < nk) {
+    sh.dph[threadIdx.x] = dph_s;
+    sh.dpx[threadIdx.x] = dpx_s;
   }
   __syncthreads();
   if (wave == 0) {
-    const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-    const double P1 = DRH * m.CP[jc];
-    const float bw = (k0 == 0) ? (float)(-(P1 * m.DPL[0])) : 0.f;
+    const float bw = (k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
@@ -730,13 +772,13 @@ void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          hipStream_t s) {
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, ta, dinv, r, yv, guard);
+    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.NI, plan.NJ, plan.NK},
+                                                 dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
-    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, ta, dinv, yv, z, guard);
+    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.NI, plan.NJ, plan.NK},
+                                                 dinv, yv, z, guard);
   }
 }
 
