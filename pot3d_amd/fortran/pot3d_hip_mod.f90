@@ -16,7 +16,9 @@ module pot3d_hip
       integer(c_int), parameter :: P3D_OPT_STOP=1, P3D_OPT_MIXED=2,     &
                                    P3D_OPT_INNER_TOL=3, P3D_OPT_CHUNK=4,&
                                    P3D_OPT_GRAPH=5, P3D_OPT_SWEEP=6,    &
-                                   P3D_OPT_VERBOSE=7
+                                   P3D_OPT_VERBOSE=7,                   &
+                                   P3D_OPT_GUESS_CHECK=8,               &
+                                   P3D_OPT_TILE_K=9
 !
       interface
 ! ****** Whole-solve ABI.
@@ -67,6 +69,43 @@ module pot3d_hip
           import :: c_int, c_ptr, c_double
           type(c_ptr), value :: h, phi, br, bt, bp
           real(c_double) :: w(3)
+        end function
+! ****** Learned warm start (no counterpart in the reference).
+        integer(c_int) function p3d_nn_create(net,nr,nt,np,rh,seed) &
+                       bind(c,name="p3d_nn_create")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr) :: net
+          integer(c_int), value :: nr,nt,np,seed
+          real(c_double) :: rh(*)
+        end function
+        integer(c_int) function p3d_nn_destroy(net) bind(c,name="p3d_nn_destroy")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: net
+        end function
+        integer(c_int) function p3d_nn_set_scales(net,bscale,phiscale) &
+                       bind(c,name="p3d_nn_set_scales")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: net
+          real(c_double), value :: bscale,phiscale
+        end function
+        integer(c_int) function p3d_nn_add_sample(net,br0,x) &
+                       bind(c,name="p3d_nn_add_sample")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: net
+          real(c_double) :: br0(*),x(*)
+        end function
+        integer(c_int) function p3d_nn_train(net,nsteps,lr,seed,losses) &
+                       bind(c,name="p3d_nn_train")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: net, losses
+          integer(c_int), value :: nsteps,seed
+          real(c_double), value :: lr
+        end function
+        integer(c_int) function p3d_nn_predict(net,br0,x,x_dev) &
+                       bind(c,name="p3d_nn_predict")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: net, x_dev
+          real(c_double) :: br0(*),x(*)
         end function
 ! ****** Legacy external LU solver ABI (src/pot3d.F90:585-603).
         subroutine load_external_lu_solver(csr_a,csr_ai,csr_aj,n,m) &
