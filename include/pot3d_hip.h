@@ -109,8 +109,11 @@ enum {
                              the warm-start network): 1 when ||b-Ax0||2 >=
                              ||b||2; 2 when its A-norm error is not smaller,
                              i.e. x0.Ax0 - 2 b.x0 >= 0 */
-  P3D_OPT_TILE_K = 9      /* planes per tile of the tile-diagonal ILU
+  P3D_OPT_TILE_K = 9,     /* planes per tile of the tile-diagonal ILU
                              schedule: 16, 32 (default) or 64 */
+  P3D_OPT_TILE_WAVES = 10 /* waves per tile of that schedule: 1 (8x8
+                             columns, default), 4 (16x16) or 8 (16x32, at
+                             most 32 planes) */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

@@ -98,6 +98,7 @@ struct p3d_solver {
   int sweep_mode = 0;
   int verbose = 0;
   int guess_check = 0;
+  int tile_k = 32, tile_waves = 1;
   // graphs: [type][0] FP64, [1] FP32
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gK[2] = {0, 0};
@@ -708,7 +709,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     hipStreamSynchronize(s->stream);
     hipFree(dpiv);
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
-    s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu) == 0);
+    s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves) == 0);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -751,16 +752,21 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
     case P3D_OPT_VERBOSE:
       s->verbose = value != 0;
       break;
-    case P3D_OPT_TILE_K: {
-      const int tk = (int)value;
-      if (tk != 16 && tk != 32 && tk != 64)
+    case P3D_OPT_TILE_K:
+    case P3D_OPT_TILE_WAVES: {
+      const int v = (int)value;
+      if (key == P3D_OPT_TILE_K && v != 16 && v != 32 && v != 64)
         return fail(P3D_EINVAL, "P3D_OPT_TILE_K must be 16, 32 or 64");
-      if (s->ifprec != PREC_ILU || tk == s->sweep.TK) break;
+      if (key == P3D_OPT_TILE_WAVES && v != 1 && v != 4 && v != 8)
+        return fail(P3D_EINVAL, "P3D_OPT_TILE_WAVES must be 1, 4 or 8");
+      if (key == P3D_OPT_TILE_K) s->tile_k = v;
+      else s->tile_waves = v;
+      if (s->ifprec != PREC_ILU) break;
       HIPCHK(hipStreamSynchronize(s->stream));
       int dev = 0, ncu = 0;
       HIPCHK(hipGetDevice(&dev));
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-      s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, tk) == 0);
+      s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves) == 0);
       for (int t = 0; t < 2; t++) {
         if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
         s->gexec[t] = nullptr;

@@ -772,25 +772,297 @@ void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          hipStream_t s) {
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.NI, plan.NJ, plan.NK},
+    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK},
                                                  dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.NI, plan.NJ, plan.NK},
+    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK},
                                                  dinv, yv, z, guard);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-wave tiles: a workgroup of SI x SJ waves owns a (8 SI) x (8 SJ) x TK
+// tile, each wave an 8x8 column stack.  Wave (si,sj) runs the same skewed
+// march 8(si+sj) steps after wave (0,0); the face values its edge lanes need
+// from the wave below in i or j were produced one step earlier and are
+// passed through LDS, with one barrier per step.  Bigger tiles mean fewer
+// anti-diagonals, i.e. fewer launches per sweep: every launch boundary costs
+// a dispatch and an L2 write-back on a multi-XCD part.
+template <int SI, int SJ, int TK>
+struct TileLdsW {
+  float v[TK][64 * SI * SJ];   // staged r (forward) / y (backward), then the result
+  float d[TK][64 * SI * SJ];   // dinv
+  float fi[TK][8 * SJ];        // incoming i face of the tile (previous launch)
+  float fj[TK][8 * SI];        // incoming j face of the tile
+  float xi[SI > 1 ? SI - 1 : 1][SJ][TK][8];  // i faces between waves
+  float xj[SI][SJ > 1 ? SJ - 1 : 1][TK][8];  // j faces between waves
+  double dph[TK], dpx[TK];
+};
+
+template <typename T, int SI, int SJ, int TK>
+__global__ void __launch_bounds__(64 * SI * SJ)
+    k_tilew_fwd(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
+                const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
+  const int done = guard ? st_done(guard) : 0;
+  __shared__ TileLdsW<SI, SJ, TK> sh;
+  int I, J, Kc;
+  diag_tile(da, blockIdx.x, I, J, Kc);
+  const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int si = wave % SI, sj = wave / SI;
+  const int ii = lane & 7, jj = lane >> 3;
+  const int i = (I * SI + si) * 8 + ii, j = (J * SJ + sj) * 8 + jj;
+  const bool act = (i < g.a) && (j < g.b);
+  const int ic = act ? i : 0, jc = act ? j : 0;
+  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
+  const double P1 = DRH * m.CP[jc];
+  const double dpu_last = m.DPU[m.c - 1];
+  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  if (act && k0 > 0) {
+    const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
+    t_own = dinv[q] * yv[q];
+  }
+  if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+  double dph_s = 0.0, dpx_s = 0.0;
+  if (tid < nk) {
+    dph_s = m.DPH[k0 + tid];
+    dpx_s = m.DPL[k0 + tid];
+  }
+  const bool face_i = (si == 0 && ii == 0 && i > 0), face_j = (sj == 0 && jj == 0 && j > 0);
+  for (int q0 = 0; q0 < TK; q0 += 16) {
+    float rv[16], dv[16], fiv[16], fjv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int q = q0 + u;
+      rv[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
+      if (act && q < nk) {
+        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
+        rv[u] = (float)r[p];
+        dv[u] = dinv[p];
+        if (face_i) fiv[u] = dinv[p - 1] * yv[p - 1];
+        if (face_j) fjv[u] = dinv[p - g.a] * yv[p - g.a];
+      }
+    }
+    if (done) return;  // uniform over the block
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int q = q0 + u;
+      if (q < nk) {
+        sh.v[q][tid] = rv[u];
+        sh.d[q][tid] = dv[u];
+        if (si == 0 && ii == 0) sh.fi[q][8 * sj + jj] = fiv[u];
+        if (sj == 0 && jj == 0) sh.fj[q][8 * si + ii] = fjv[u];
+      }
+    }
+  }
+  if (tid < nk) {
+    sh.dph[tid] = dph_s;
+    sh.dpx[tid] = dpx_s;
+  }
+  __syncthreads();
+  const float aw = (k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
+  const int off = 8 * (si + sj);
+  const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
+  for (int T0 = 0; T0 < S; T0 += 8) {
+    float rr[8], dd[8], ak[8], aj[8], ai[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = T0 + u - off - ii - jj;
+      rr[u] = dd[u] = ak[u] = aj[u] = ai[u] = 0.f;
+      if (act && q >= 0 && q < nk) {
+        rr[u] = sh.v[q][tid];
+        dd[u] = sh.d[q][tid];
+        const double dph = sh.dph[q];
+        ak[u] = (float)(-(P1 * sh.dpx[q]));
+        aj[u] = (float)(-((DRH * dph) * CTL));
+        ai[u] = (float)(-((SD * dph) * CRL));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = T0 + u - off - ii - jj;
+      const int k = k0 + q;
+      const float from_i = __shfl_up(t_pub, 1, 64);
+      const float from_j = __shfl_up(t_pub, 8, 64);
+      if (act && q >= 0 && q < nk) {
+        float fl = from_i, fd = from_j;
+        if (ii == 0) fl = (si == 0) ? sh.fi[q][8 * sj + jj] : sh.xi[si > 0 ? si - 1 : 0][sj][q][jj];
+        if (jj == 0) fd = (sj == 0) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj > 0 ? sj - 1 : 0][q][ii];
+        float y = rr[u];
+        if (k == g.c - 1) y = y - aw * t_w;
+        if (k > 0) y = y - ak[u] * t_own;
+        if (j > 0) y = y - aj[u] * fd;
+        if (i > 0) y = y - ai[u] * fl;
+        sh.v[q][tid] = y;
+        const float tt = dd[u] * y;
+        if (ii == 7 && si < SI - 1) sh.xi[si][sj][q][jj] = tt;
+        if (jj == 7 && sj < SJ - 1) sh.xj[si][sj][q][ii] = tt;
+        if (k == 0) t_w = tt;
+        t_own = tt;
+        t_pub = tt;
+      }
+      __syncthreads();
+    }
+  }
+  if (act)
+    for (int q = 0; q < nk; q++) yv[col + g.ab * (uint32_t)(k0 + q)] = sh.v[q][tid];
+}
+
+template <typename T, int SI, int SJ, int TK>
+__global__ void __launch_bounds__(64 * SI * SJ)
+    k_tilew_bwd(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
+                float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
+  const int done = guard ? st_done(guard) : 0;
+  __shared__ TileLdsW<SI, SJ, TK> sh;
+  int I, J, Kc;
+  diag_tile(da, blockIdx.x, I, J, Kc);
+  const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int si = wave % SI, sj = wave / SI;
+  const int ii = lane & 7, jj = lane >> 3;
+  const int rii = 7 - ii, rjj = 7 - jj;
+  const int i = (I * SI + si) * 8 + ii, j = (J * SJ + sj) * 8 + jj;
+  const bool act = (i < g.a) && (j < g.b);
+  const int ic = act ? i : 0, jc = act ? j : 0;
+  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
+  const double P1 = DRH * m.CP[jc];
+  const double dpl_first = m.DPL[0];
+  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
+  if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  double dph_s = 0.0, dpx_s = 0.0;
+  if (tid < nk) {
+    dph_s = m.DPH[k0 + tid];
+    dpx_s = m.DPU[k0 + tid];
+  }
+  const bool face_i = (si == SI - 1 && ii == 7 && i < g.a - 1);
+  const bool face_j = (sj == SJ - 1 && jj == 7 && j < g.b - 1);
+  for (int q0 = 0; q0 < TK; q0 += 16) {
+    float yv16[16], dv[16], fiv[16], fjv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int q = q0 + u;
+      yv16[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
+      if (act && q < nk) {
+        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
+        yv16[u] = yv[p];
+        dv[u] = dinv[p];
+        if (face_i) fiv[u] = yv[p + 1];
+        if (face_j) fjv[u] = yv[p + g.a];
+      }
+    }
+    if (done) return;  // uniform over the block
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int q = q0 + u;
+      if (q < nk) {
+        sh.v[q][tid] = yv16[u];
+        sh.d[q][tid] = dv[u];
+        if (si == SI - 1 && ii == 7) sh.fi[q][8 * sj + jj] = fiv[u];
+        if (sj == SJ - 1 && jj == 7) sh.fj[q][8 * si + ii] = fjv[u];
+      }
+    }
+  }
+  if (tid < nk) {
+    sh.dph[tid] = dph_s;
+    sh.dpx[tid] = dpx_s;
+  }
+  __syncthreads();
+  const float bw = (k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
+  const int off = 8 * ((SI - 1 - si) + (SJ - 1 - sj));
+  const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
+  for (int T0 = 0; T0 < S; T0 += 8) {
+    float yy[8], dd[8], bk[8], bj[8], bi[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = (nk - 1) - (T0 + u - off - rii - rjj);
+      yy[u] = dd[u] = bk[u] = bj[u] = bi[u] = 0.f;
+      if (act && q >= 0 && q < nk) {
+        yy[u] = sh.v[q][tid];
+        dd[u] = sh.d[q][tid];
+        const double dph = sh.dph[q];
+        bk[u] = (float)(-(P1 * sh.dpx[q]));
+        bj[u] = (float)(-((DRH * dph) * CTH));
+        bi[u] = (float)(-((SD * dph) * CRH));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = (nk - 1) - (T0 + u - off - rii - rjj);
+      const int k = k0 + q;
+      const float from_i = __shfl_down(t_pub, 1, 64);
+      const float from_j = __shfl_down(t_pub, 8, 64);
+      if (act && q >= 0 && q < nk) {
+        float fr = from_i, fu = from_j;
+        if (ii == 7) fr = (si == SI - 1) ? sh.fi[q][8 * sj + jj] : sh.xi[si < SI - 1 ? si : 0][sj][q][jj];
+        if (jj == 7) fu = (sj == SJ - 1) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj < SJ - 1 ? sj : 0][q][ii];
+        float sv = yy[u];
+        if (i < g.a - 1) sv = sv - bi[u] * fr;
+        if (j < g.b - 1) sv = sv - bj[u] * fu;
+        if (k < g.c - 1) sv = sv - bk[u] * t_own;
+        if (k == 0) sv = sv - bw * t_w;
+        const float zz = sv * dd[u];
+        sh.v[q][tid] = zz;
+        if (ii == 0 && si > 0) sh.xi[si - 1][sj][q][jj] = zz;
+        if (jj == 0 && sj > 0) sh.xj[si][sj - 1][q][ii] = zz;
+        if (k == g.c - 1) t_w = zz;
+        t_own = zz;
+        t_pub = zz;
+      }
+      __syncthreads();
+    }
+  }
+  if (act)
+    for (int q = 0; q < nk; q++) {
+      const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
+      const float zz = sh.v[q][tid];
+      yv[p] = zz;
+      z[p] = (T)zz;
+    }
+}
+
+template <typename T, int SI, int SJ, int TK>
+void tiles_launch_w(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
+                    const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
+  for (int L = 0; L < plan.ndiag; L++) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    k_tilew_fwd<T, SI, SJ, TK><<<cnt, 64 * SI * SJ, 0, s>>>(
+        m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK}, dinv, r, yv, guard);
+  }
+  for (int L = plan.ndiag - 1; L >= 0; L--) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    k_tilew_bwd<T, SI, SJ, TK><<<cnt, 64 * SI * SJ, 0, s>>>(
+        m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK}, dinv, yv, z, guard);
   }
 }
 
 template <typename T>
 void tiles_launch_lds(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
                       const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
-  if (plan.TK == 16)
+  const int w = plan.SI * plan.SJ;
+  if (w == 4) {
+    if (plan.TK == 16)
+      tiles_launch_w<T, 2, 2, 16>(plan, m, g, dinv, r, yv, z, guard, s);
+    else if (plan.TK == 64)
+      tiles_launch_w<T, 2, 2, 64>(plan, m, g, dinv, r, yv, z, guard, s);
+    else
+      tiles_launch_w<T, 2, 2, 32>(plan, m, g, dinv, r, yv, z, guard, s);
+  } else if (w == 8) {
+    if (plan.TK == 16)
+      tiles_launch_w<T, 2, 4, 16>(plan, m, g, dinv, r, yv, z, guard, s);
+    else
+      tiles_launch_w<T, 2, 4, 32>(plan, m, g, dinv, r, yv, z, guard, s);
+  } else if (plan.TK == 16) {
     tiles_launch_lds_tk<T, 16>(plan, m, g, dinv, r, yv, z, guard, s);
-  else if (plan.TK == 64)
+  } else if (plan.TK == 64) {
     tiles_launch_lds_tk<T, 64>(plan, m, g, dinv, r, yv, z, guard, s);
-  else
+  } else {
     tiles_launch_lds_tk<T, 32>(plan, m, g, dinv, r, yv, z, guard, s);
+  }
 }
 
 template <typename T, typename CS>
@@ -798,19 +1070,19 @@ void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const flo
                   const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
+    TileArgs ta{plan.TNI, plan.TNJ, plan.TK, plan.tiles + plan.diag_off[L]};
     k_tile_fwd<T, CS><<<cnt, 64, 0, s>>>(cs, g, ta, dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    TileArgs ta{plan.NI, plan.NJ, plan.TK, plan.tiles + plan.diag_off[L]};
+    TileArgs ta{plan.TNI, plan.TNJ, plan.TK, plan.tiles + plan.diag_off[L]};
     k_tile_bwd<T, CS><<<cnt, 64, 0, s>>>(cs, g, ta, dinv, yv, z, guard);
   }
 }
 
 }  // namespace
 
-int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k) {
+int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   plan.NI = (g.a + TI - 1) / TI;
@@ -851,16 +1123,25 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k) {
   }
   // tile diagonals
   plan.TK = (tile_k == 16 || tile_k == 64) ? tile_k : kTileK;
+  plan.SI = plan.SJ = 1;
+  if (tile_waves == 4) plan.SI = plan.SJ = 2;
+  if (tile_waves == 8) {
+    plan.SI = 2;
+    plan.SJ = 4;
+    if (plan.TK == 64) plan.TK = kTileK;  // the LDS holds at most 16x32x32
+  }
+  plan.TNI = (g.a + TI * plan.SI - 1) / (TI * plan.SI);
+  plan.TNJ = (g.b + TJ * plan.SJ - 1) / (TJ * plan.SJ);
   plan.NK = (g.c + plan.TK - 1) / plan.TK;
-  plan.ndiag = plan.NI + plan.NJ + plan.NK - 2;
+  plan.ndiag = plan.TNI + plan.TNJ + plan.NK - 2;
   std::vector<int> tl;
   plan.diag_off.assign(plan.ndiag + 1, 0);
   for (int L = 0; L < plan.ndiag; L++) {
     plan.diag_off[L] = (int)tl.size();
     for (int Kc = 0; Kc < plan.NK; Kc++)
-      for (int J = 0; J < plan.NJ; J++) {
+      for (int J = 0; J < plan.TNJ; J++) {
         const int I = L - Kc - J;
-        if (I >= 0 && I < plan.NI) tl.push_back(I + plan.NI * (J + plan.NJ * Kc));
+        if (I >= 0 && I < plan.TNI) tl.push_back(I + plan.TNI * (J + plan.TNJ * Kc));
       }
   }
   plan.diag_off[plan.ndiag] = (int)tl.size();

@@ -38,15 +38,19 @@ struct SweepPlan {
   unsigned* prog_b = nullptr;   // per-item planes finished (backward)
   SweepCtl* ctl = nullptr;
   double* item_partial = nullptr;  // per-tile partial r.z of the backward sweep
-  // tile-diagonal schedule (8x8xTK tiles, one launch per anti-diagonal)
+  // tile-diagonal schedule ((8 SI) x (8 SJ) x TK tiles, one launch per
+  // anti-diagonal; SI x SJ waves per tile)
   int TK = 0, NK = 0, ndiag = 0;
+  int SI = 1, SJ = 1, TNI = 0, TNJ = 0;
   int* tiles = nullptr;            // tiles ordered by diagonal
   std::vector<int> diag_off;       // ndiag+1 offsets into tiles
 };
 
 // Returns 0 when the dataflow sweep can run on this grid.
 // tile_k: planes per tile of the tile-diagonal schedule (16, 32 or 64).
-int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32);
+// tile_waves: waves per tile, 1 (8x8), 4 (16x16) or 8 (16x32; tile_k <= 32).
+int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32,
+                     int tile_waves = 1);
 void free_sweep_plan(SweepPlan& plan);
 // 1 if a sweep wait exceeded its bound since the last call (and clears it).
 int sweep_error(const SweepPlan& plan);

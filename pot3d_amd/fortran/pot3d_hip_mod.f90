@@ -18,7 +18,7 @@ module pot3d_hip
                                    P3D_OPT_GRAPH=5, P3D_OPT_SWEEP=6,    &
                                    P3D_OPT_VERBOSE=7,                   &
                                    P3D_OPT_GUESS_CHECK=8,               &
-                                   P3D_OPT_TILE_K=9
+                                   P3D_OPT_TILE_K=9, P3D_OPT_TILE_WAVES=10
 !
       interface
 ! ****** Whole-solve ABI.

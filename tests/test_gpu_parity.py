@@ -134,11 +134,14 @@ def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
             assert np.array_equal(z_df, z_df2)
             assert np.array_equal(z_tl, z_lv), (rep, np.max(np.abs(z_tl - z_lv)))
             assert np.array_equal(z_tl, z_tl2)
-            # tile depths 16 and 64 (32 is the default above)
-            for tk in (16, 64):
+            # other tile shapes: waves per tile x planes per tile
+            for waves, tk in ((1, 16), (1, 64), (4, 16), (4, 32), (4, 64), (8, 16), (8, 32)):
+                s.set_option(_lib.OPT_TILE_WAVES, waves)
                 s.set_option(_lib.OPT_TILE_K, tk)
                 z_tk = s.prec_inv(x)
-                assert np.array_equal(z_tk, z_lv), (rep, tk, np.max(np.abs(z_tk - z_lv)))
+                assert np.array_equal(z_tk, z_lv), (rep, waves, tk,
+                                                    np.max(np.abs(z_tk - z_lv)))
+            s.set_option(_lib.OPT_TILE_WAVES, 1)
             s.set_option(_lib.OPT_TILE_K, 32)
 
 
@@ -168,6 +171,24 @@ def test_ilu_solve_same_with_both_sweeps(solver_mod):
 
 def _rel(a, b):
     return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def test_ilu_solve_multiwave_tiles(solver_mod):
+    """A whole PCG solve with 16x32-column tiles equals the default tiles."""
+    say("[gpu] ilu solve multi-wave tiles")
+    from pot3d_amd import _lib
+    mesh = small_mesh(33, 45, 70)
+    br0 = random_br0(mesh, 5)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+        s.set_br0(br0)
+        x1 = np.zeros(s.N)
+        assert s.solve(x1, None, 1e-9, ncgmax=2000)[0] == 0
+        n1 = s.stats()["ncg"]
+        for waves in (4, 8):
+            s.set_option(_lib.OPT_TILE_WAVES, waves)
+            x2 = np.zeros(s.N)
+            assert s.solve(x2, None, 1e-9, ncgmax=2000)[0] == 0
+            assert s.stats()["ncg"] == n1 and np.array_equal(x1, x2)
 
 
 def test_ilu_solve_tiles_vs_hyperplanes(solver_mod):
