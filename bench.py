@@ -93,6 +93,7 @@ def cpu_baseline(workload, gpu_iters: int, sample_iters: int, threads: int):
 
 # (name, time_kernel id, algorithmic bytes per unknown, padded-array bytes per point)
 KERNELS = [
+    ("pcg_iteration_fp64 (graph replay)", 3, 136, 8),
     ("stencil_fp64 (k_stencil_march<double>)", 0, 8, 8),
     ("stencil_fp32 (k_stencil_march<float>)", 1, 4, 4),
     ("ilu_apply_fp64 (k_tile_fwd+k_tile_bwd<double>)", 2, 32, 0),

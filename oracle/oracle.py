@@ -62,6 +62,8 @@ def lib():
             "orc_lusol": (None, [C.c_int64, _fp, _fp, _ip, _ip, _ip, _fp]),
             "orc_dilu_pivots": (None, [G, _dp, _dp]),
             "orc_dilu_apply": (None, [G, _dp, _fp, _fp]),
+            "orc_dilu_pivots_blk": (None, [G, _dp, C.c_int, C.c_int, C.c_int, _dp]),
+            "orc_dilu_apply_blk": (None, [G, _dp, _fp, C.c_int, C.c_int, C.c_int, _fp]),
             "orc_prec_apply": (C.c_int, [G, _dp, C.c_int, _dp]),
             "orc_cgsolve": (C.c_int, [G, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp,
                                       _ip, _dp, _dp, C.c_int]),
@@ -203,20 +205,32 @@ class Oracle:
         lib().orc_lusol(self.N, _f(x), _f(LU), _i(LUJA), _i(N1), _i(N2), _f(lud))
         return x
 
-    def dilu_pivots(self) -> np.ndarray:
+    def dilu_pivots(self, blocks=None) -> np.ndarray:
+        """D-ILU pivots; blocks=(bi, bj, bk) block extents in points selects
+        the block-Jacobi variant (couplings between blocks dropped)."""
         d = np.empty(self.N)
-        lib().orc_dilu_pivots(self.gp, _d(self.coefs().ravel()), _d(d))
+        if blocks is None:
+            lib().orc_dilu_pivots(self.gp, _d(self.coefs().ravel()), _d(d))
+        else:
+            bi, bj, bk = (int(v) for v in blocks)
+            lib().orc_dilu_pivots_blk(self.gp, _d(self.coefs().ravel()), bi, bj, bk, _d(d))
         return d
 
-    def dilu_dinv(self) -> np.ndarray:
-        return (1.0 / self.dilu_pivots()).astype(np.float32)
+    def dilu_dinv(self, blocks=None) -> np.ndarray:
+        return (1.0 / self.dilu_pivots(blocks)).astype(np.float32)
 
-    def dilu_apply(self, r32: np.ndarray, dinv: np.ndarray | None = None) -> np.ndarray:
+    def dilu_apply(self, r32: np.ndarray, dinv: np.ndarray | None = None,
+                   blocks=None) -> np.ndarray:
         if dinv is None:
-            dinv = self.dilu_dinv()
+            dinv = self.dilu_dinv(blocks)
         x = np.array(r32, dtype=np.float32)
-        lib().orc_dilu_apply(self.gp, _d(self.coefs().ravel()), _f(np.ascontiguousarray(dinv)),
-                             _f(x))
+        if blocks is None:
+            lib().orc_dilu_apply(self.gp, _d(self.coefs().ravel()),
+                                 _f(np.ascontiguousarray(dinv)), _f(x))
+        else:
+            bi, bj, bk = (int(v) for v in blocks)
+            lib().orc_dilu_apply_blk(self.gp, _d(self.coefs().ravel()),
+                                     _f(np.ascontiguousarray(dinv)), bi, bj, bk, _f(x))
         return x
 
     def prec_apply(self, x: np.ndarray, ifprec: int) -> np.ndarray:

@@ -380,29 +380,41 @@ void orc_lusol(int64_t N, float *x, const float *LU, const int *LUJA, const int 
  * pivots change:  d_p = a4_p - sum_{q in lower(p)} (a_pq/d_q)*a_qp,
  * summed in the column order ilu0 uses (wrap, k-1, j-1, i-1).
  * FP64 here; the product path stores dinv = real(1/d, REAL32).          */
-void orc_dilu_pivots(const orc_grid *g, const double *a, double *d) {
+/* Block-Jacobi variant: the grid is cut into blocks of bi x bj x bk points
+ * (block starts at multiples of bi, bj, bk) and every coupling between two
+ * blocks is dropped from the ILU matrix -- the reference's per-rank ILU0 on
+ * a decomposed grid (diacsr drops the couplings at each rank's local r and
+ * theta boundaries, and the phi ones when nproc_p > 1, :4940-4967).  The
+ * phi wrap is kept only when there is a single phi block (bk >= C).
+ * bi >= A, bj >= B, bk >= C is the unblocked D-ILU above.                */
+static int same_lo(int x, int b) { return x > 0 && (x % b) != 0; }
+static int same_hi(int x, int n, int b) { return x < n - 1 && ((x + 1) % b) != 0; }
+
+void orc_dilu_pivots_blk(const orc_grid *g, const double *a, int bi, int bj, int bk,
+                         double *d) {
   const int nr = g->nr, nt = g->nt, np = g->np;
   const int A = nr - 2, B = nt - 2, C = np - 2;
   const size_t N = (size_t)A * B * C;
   const size_t sj = (size_t)A, sk = (size_t)A * B;
+  const int wrap = bk >= C;
   for (int k = 0; k < C; k++)
     for (int j = 0; j < B; j++)
       for (int i = 0; i < A; i++) {
         const size_t p = i + sj * j + sk * k;
         double dp_ = a[3 * N + p];
-        if (k == C - 1) { /* wrap: q=(i,j,0); a_pq = a7(p), a_qp = a1(q) */
+        if (wrap && k == C - 1) { /* wrap: q=(i,j,0); a_pq = a7(p), a_qp = a1(q) */
           const size_t q = i + sj * j;
           dp_ = dp_ - a[6 * N + p] / d[q] * a[0 * N + q];
         }
-        if (k > 0) { /* q=(i,j,k-1): a_pq = a1(p), a_qp = a7(q) */
+        if (same_lo(k, bk)) { /* q=(i,j,k-1): a_pq = a1(p), a_qp = a7(q) */
           const size_t q = p - sk;
           dp_ = dp_ - a[0 * N + p] / d[q] * a[6 * N + q];
         }
-        if (j > 0) { /* q=(i,j-1,k): a2(p), a6(q) */
+        if (same_lo(j, bj)) { /* q=(i,j-1,k): a2(p), a6(q) */
           const size_t q = p - sj;
           dp_ = dp_ - a[1 * N + p] / d[q] * a[5 * N + q];
         }
-        if (i > 0) { /* q=(i-1,j,k): a3(p), a5(q) */
+        if (same_lo(i, bi)) { /* q=(i-1,j,k): a3(p), a5(q) */
           const size_t q = p - 1;
           dp_ = dp_ - a[2 * N + p] / d[q] * a[4 * N + q];
         }
@@ -410,24 +422,31 @@ void orc_dilu_pivots(const orc_grid *g, const double *a, double *d) {
       }
 }
 
+void orc_dilu_pivots(const orc_grid *g, const double *a, double *d) {
+  orc_dilu_pivots_blk(g, a, g->nr, g->nt, g->np, d);
+}
+
 /* D-ILU apply in FP32, the arithmetic of the product's trisolve kernels:
  *   forward  y_p = r_p - sum_{q<p} a_pq * (dinv_q * y_q)   (wrap,k-1,j-1,i-1)
  *   backward z_p = (y_p - sum_{q>p} a_pq * z_q) * dinv_p    (i+1,j+1,k+1,wrap)
- * with a_pq = real(a, REAL32).  x is overwritten (in: r, out: z).       */
-void orc_dilu_apply(const orc_grid *g, const double *a, const float *dinv, float *x) {
+ * with a_pq = real(a, REAL32); couplings between blocks dropped as in
+ * orc_dilu_pivots_blk.  x is overwritten (in: r, out: z).                */
+void orc_dilu_apply_blk(const orc_grid *g, const double *a, const float *dinv, int bi, int bj,
+                        int bk, float *x) {
   const int nr = g->nr, nt = g->nt, np = g->np;
   const int A = nr - 2, B = nt - 2, C = np - 2;
   const size_t N = (size_t)A * B * C;
   const size_t sj = (size_t)A, sk = (size_t)A * B;
+  const int wrap = bk >= C;
   for (int k = 0; k < C; k++)
     for (int j = 0; j < B; j++)
       for (int i = 0; i < A; i++) {
         const size_t p = i + sj * j + sk * k;
         float y = x[p];
-        if (k == C - 1) y = y - (float)a[6 * N + p] * (dinv[i + sj * j] * x[i + sj * j]);
-        if (k > 0) y = y - (float)a[0 * N + p] * (dinv[p - sk] * x[p - sk]);
-        if (j > 0) y = y - (float)a[1 * N + p] * (dinv[p - sj] * x[p - sj]);
-        if (i > 0) y = y - (float)a[2 * N + p] * (dinv[p - 1] * x[p - 1]);
+        if (wrap && k == C - 1) y = y - (float)a[6 * N + p] * (dinv[i + sj * j] * x[i + sj * j]);
+        if (same_lo(k, bk)) y = y - (float)a[0 * N + p] * (dinv[p - sk] * x[p - sk]);
+        if (same_lo(j, bj)) y = y - (float)a[1 * N + p] * (dinv[p - sj] * x[p - sj]);
+        if (same_lo(i, bi)) y = y - (float)a[2 * N + p] * (dinv[p - 1] * x[p - 1]);
         x[p] = y;
       }
   for (int k = C - 1; k >= 0; k--)
@@ -435,12 +454,16 @@ void orc_dilu_apply(const orc_grid *g, const double *a, const float *dinv, float
       for (int i = A - 1; i >= 0; i--) {
         const size_t p = i + sj * j + sk * k;
         float s = x[p];
-        if (i < A - 1) s = s - (float)a[4 * N + p] * x[p + 1];
-        if (j < B - 1) s = s - (float)a[5 * N + p] * x[p + sj];
-        if (k < C - 1) s = s - (float)a[6 * N + p] * x[p + sk];
-        if (k == 0) s = s - (float)a[0 * N + p] * x[i + sj * j + sk * (C - 1)];
+        if (same_hi(i, A, bi)) s = s - (float)a[4 * N + p] * x[p + 1];
+        if (same_hi(j, B, bj)) s = s - (float)a[5 * N + p] * x[p + sj];
+        if (same_hi(k, C, bk)) s = s - (float)a[6 * N + p] * x[p + sk];
+        if (wrap && k == 0) s = s - (float)a[0 * N + p] * x[i + sj * j + sk * (C - 1)];
         x[p] = s * dinv[p];
       }
+}
+
+void orc_dilu_apply(const orc_grid *g, const double *a, const float *dinv, float *x) {
+  orc_dilu_apply_blk(g, a, dinv, g->nr, g->nt, g->np, x);
 }
 
 /* ---------------------------------------------------------------------- */

@@ -82,6 +82,24 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
   return (__umulhi(n, f.mul) + n) >> f.shift;
 }
 
+// Block-Jacobi cut of the ILU matrix: blocks of bi x bj x bk points with
+// block starts at multiples of the extents; every coupling between two
+// blocks is dropped, and the phi wrap is kept only with a single phi block.
+// This is the ILU0 the reference builds on a decomposed grid: each rank's
+// diacsr drops the couplings at its local r and theta boundaries and, when
+// nproc_p > 1, the phi ones (src/pot3d.F90:4940-4967).  Extents >= (a,b,c):
+// no cut.
+struct IluBlocks {
+  int bi, bj, bk;
+  int wrap;  // phi wrap kept
+  __device__ __forceinline__ bool lo_i(int i) const { return i > 0 && i % bi != 0; }
+  __device__ __forceinline__ bool lo_j(int j) const { return j > 0 && j % bj != 0; }
+  __device__ __forceinline__ bool lo_k(int k) const { return k > 0 && k % bk != 0; }
+  __device__ __forceinline__ bool hi_i(int i, int a) const { return i < a - 1 && (i + 1) % bi != 0; }
+  __device__ __forceinline__ bool hi_j(int j, int b) const { return j < b - 1 && (j + 1) % bj != 0; }
+  __device__ __forceinline__ bool hi_k(int k, int c) const { return k < c - 1 && (k + 1) % bk != 0; }
+};
+
 struct Grid3 {
   int a, b, c;
   uint32_t N, ab;

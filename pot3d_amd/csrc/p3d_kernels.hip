@@ -687,7 +687,7 @@ __device__ __forceinline__ bool level_point(const Grid3& g, int L, int kmin, int
 }
 
 template <typename CP>
-__global__ void k_dilu_factor_level(CP cp, Grid3 g, int L, int kmin, int wrap,
+__global__ void k_dilu_factor_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks bl,
                                     double* __restrict__ d, float* __restrict__ dinv) {
   int i, j, k;
   if (!level_point(g, L, kmin, i, j, k)) return;
@@ -700,9 +700,9 @@ __global__ void k_dilu_factor_level(CP cp, Grid3 g, int L, int kmin, int wrap,
     const uint32_t q = (uint32_t)i + (uint32_t)g.a * (uint32_t)j;
     dp = dp - aw / d[q] * tw;
   }
-  if (k > 0) dp = dp - ak / d[n - g.ab] * tk;
-  if (j > 0) dp = dp - aj / d[n - (uint32_t)g.a] * tj;
-  if (i > 0) dp = dp - ai / d[n - 1] * ti;
+  if (bl.lo_k(k)) dp = dp - ak / d[n - g.ab] * tk;
+  if (bl.lo_j(j)) dp = dp - aj / d[n - (uint32_t)g.a] * tj;
+  if (bl.lo_i(i)) dp = dp - ai / d[n - 1] * ti;
   d[n] = dp;
   dinv[n] = (float)(1.0 / dp);
 }
@@ -714,22 +714,27 @@ static void level_grid(const Grid3& g, int L, dim3& grid, int& kmin) {
   grid = dim3((g.b + 63) / 64, (unsigned)(kmax - kmin + 1), 1);
 }
 
+static IluBlocks no_blocks(const Grid3& g, bool wrap) {
+  return IluBlocks{g.a, g.b, g.c, wrap ? 1 : 0};
+}
+
 template <typename CP>
-static void dilu_factor_levels(const CP& cp, const Grid3& g, bool wrap, double* d, float* dinv,
-                               hipStream_t s) {
+static void dilu_factor_levels(const CP& cp, const Grid3& g, const IluBlocks& bl, double* d,
+                               float* dinv, hipStream_t s) {
   const int nlev = g.a + g.b + g.c - 2;
   for (int L = 0; L < nlev; L++) {
     dim3 grid;
     int kmin;
     level_grid(g, L, grid, kmin);
-    k_dilu_factor_level<CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, wrap ? 1 : 0, d, dinv);
+    k_dilu_factor_level<CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, bl.wrap, bl, d, dinv);
   }
 }
 
-void launch_dilu_factor(const Metric& m, const Grid3& g, double* d, float* dinv, hipStream_t s) {
+void launch_dilu_factor(const Metric& m, const Grid3& g, const IluBlocks& bl, double* d,
+                        float* dinv, hipStream_t s) {
   MFCoefs cp;
   cp.m = m;
-  dilu_factor_levels(cp, g, true, d, dinv, s);
+  dilu_factor_levels(cp, g, bl, d, dinv, s);
 }
 
 void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid3& g, bool wrap,
@@ -741,13 +746,13 @@ void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid
   cp.b = g.b;
   cp.c = g.c;
   cp.ab = g.ab;
-  dilu_factor_levels(cp, g, wrap, d, dinv, s);
+  dilu_factor_levels(cp, g, no_blocks(g, wrap), d, dinv, s);
 }
 
 // Forward sweep on one hyperplane:
 //   y_p = r_p - sum_{q<p} a_pq (dinv_q y_q)    (order: wrap, k-1, j-1, i-1)
 template <typename T, typename CP>
-__global__ void k_fwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
+__global__ void k_fwd_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks bl,
                             const float* __restrict__ dinv, const T* __restrict__ r,
                             float* __restrict__ yv, const CgState* guard) {
   if (guard && st_done(guard)) return;
@@ -761,16 +766,16 @@ __global__ void k_fwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
     const uint32_t q = (uint32_t)i + (uint32_t)g.a * (uint32_t)j;
     y = y - (float)aw * (dinv[q] * yv[q]);
   }
-  if (k > 0) y = y - (float)ak * (dinv[n - g.ab] * yv[n - g.ab]);
-  if (j > 0) y = y - (float)aj * (dinv[n - g.a] * yv[n - g.a]);
-  if (i > 0) y = y - (float)ai * (dinv[n - 1] * yv[n - 1]);
+  if (bl.lo_k(k)) y = y - (float)ak * (dinv[n - g.ab] * yv[n - g.ab]);
+  if (bl.lo_j(j)) y = y - (float)aj * (dinv[n - g.a] * yv[n - g.a]);
+  if (bl.lo_i(i)) y = y - (float)ai * (dinv[n - 1] * yv[n - 1]);
   yv[n] = y;
 }
 
 // Backward sweep on one hyperplane (in place over yv):
 //   z_p = (y_p - sum_{q>p} a_pq z_q) dinv_p   (order: i+1, j+1, k+1, wrap)
 template <typename T, typename CP>
-__global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
+__global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks bl,
                             const float* __restrict__ dinv, float* __restrict__ yv,
                             T* __restrict__ z, const CgState* guard) {
   if (guard && st_done(guard)) return;
@@ -780,9 +785,9 @@ __global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
   double bi, bj, bk, bw;
   cp.upper(i, j, k, n, bi, bj, bk, bw);
   float s = yv[n];
-  if (i < g.a - 1) s = s - (float)bi * yv[n + 1];
-  if (j < g.b - 1) s = s - (float)bj * yv[n + g.a];
-  if (k < g.c - 1) s = s - (float)bk * yv[n + g.ab];
+  if (bl.hi_i(i, g.a)) s = s - (float)bi * yv[n + 1];
+  if (bl.hi_j(j, g.b)) s = s - (float)bj * yv[n + g.a];
+  if (bl.hi_k(k, g.c)) s = s - (float)bk * yv[n + g.ab];
   if (wrap && k == 0 && g.c > 1) s = s - (float)bw * yv[n + (uint32_t)(g.c - 1) * g.ab];
   const float zz = s * dinv[n];
   yv[n] = zz;
@@ -790,27 +795,28 @@ __global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap,
 }
 
 template <typename T, typename CP>
-static void dilu_solve_levels(const CP& cp, const Grid3& g, bool wrap, const float* dinv,
-                              const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
+static void dilu_solve_levels(const CP& cp, const Grid3& g, const IluBlocks& bl,
+                              const float* dinv, const T* r, float* yv, T* z,
+                              const CgState* guard, hipStream_t s) {
   const int nlev = g.a + g.b + g.c - 2;
   for (int L = 0; L < nlev; L++) {
     dim3 grid;
     int kmin;
     level_grid(g, L, grid, kmin);
-    k_fwd_level<T, CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, wrap ? 1 : 0, dinv, r, yv, guard);
+    k_fwd_level<T, CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, bl.wrap, bl, dinv, r, yv, guard);
   }
   for (int L = nlev - 1; L >= 0; L--) {
     dim3 grid;
     int kmin;
     level_grid(g, L, grid, kmin);
-    k_bwd_level<T, CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, wrap ? 1 : 0, dinv, yv, z, guard);
+    k_bwd_level<T, CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, bl.wrap, bl, dinv, yv, z, guard);
   }
 }
 
 template <typename T>
 void launch_dilu_solve_levels(const Metric& m, const Grid3& g, const float* dinv, const T* r,
                               float* yv, T* z, const CoefArrays* ca, bool wrap,
-                              const CgState* guard, hipStream_t s) {
+                              const IluBlocks& bl, const CgState* guard, hipStream_t s) {
   if (ca) {
     StoredCoefs cp;
     cp.ca = *ca;
@@ -819,11 +825,11 @@ void launch_dilu_solve_levels(const Metric& m, const Grid3& g, const float* dinv
     cp.b = g.b;
     cp.c = g.c;
     cp.ab = g.ab;
-    dilu_solve_levels<T>(cp, g, wrap, dinv, r, yv, z, guard, s);
+    dilu_solve_levels<T>(cp, g, no_blocks(g, wrap), dinv, r, yv, z, guard, s);
   } else {
     MFCoefs cp;
     cp.m = m;
-    dilu_solve_levels<T>(cp, g, true, dinv, r, yv, z, guard, s);
+    dilu_solve_levels<T>(cp, g, bl, dinv, r, yv, z, guard, s);
   }
 }
 
@@ -987,7 +993,7 @@ void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi
   template void launch_copy<T>(const Grid3&, const Launch&, T*, const T*, hipStream_t);         \
   template void launch_dilu_solve_levels<T>(const Metric&, const Grid3&, const float*,          \
                                             const T*, float*, T*, const CoefArrays*, bool,      \
-                                            const CgState*, hipStream_t);
+                                            const IluBlocks&, const CgState*, hipStream_t);
 P3D_INST(double)
 P3D_INST(float)
 template void launch_cast_scale<float, double>(uint32_t, float*, const double*, const double*,

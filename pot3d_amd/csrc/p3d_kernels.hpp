@@ -156,9 +156,10 @@ void launch_fill_f(float* p, float v, size_t n, hipStream_t s);
 void launch_rhs(const Metric& m, const Grid3& g, const double* br0, const double* br1,
                 double dr1, double drn, int nt, double* rhs, hipStream_t s);
 
-// D-ILU pivots (FP64, hyperplane level launches) -> dinv (FP32).
-void launch_dilu_factor(const Metric& m, const Grid3& g, double* d, float* dinv,
-                        hipStream_t s);
+// D-ILU pivots (FP64, hyperplane level launches) -> dinv (FP32), with the
+// block-Jacobi cut bl.
+void launch_dilu_factor(const Metric& m, const Grid3& g, const IluBlocks& bl, double* d,
+                        float* dinv, hipStream_t s);
 
 // Stored couplings of the legacy CSR input (FP32, N each unless noted):
 // lo_k/lo_j/lo_i couple p to (i,j,k-1)/(i,j-1,k)/(i-1,j,k), up_i/up_j/up_k
@@ -177,10 +178,11 @@ struct CoefArrays {
 };
 
 // Level-scheduled D-ILU sweeps: r (T) -> z (T); yv is FP32 scratch of N.
+// bl: block-Jacobi cut (matrix-free coefficients only; ca != null ignores it).
 template <typename T>
 void launch_dilu_solve_levels(const Metric& m, const Grid3& g, const float* dinv, const T* r,
                               float* yv, T* z, const CoefArrays* ca, bool wrap,
-                              const CgState* guard, hipStream_t s);
+                              const IluBlocks& bl, const CgState* guard, hipStream_t s);
 
 void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid3& g, bool wrap,
                                double* d, float* dinv, hipStream_t s);

@@ -301,7 +301,8 @@ void external_lu_solver(double* x) {
   } else {
     Metric m;
     memset(&m, 0, sizeof m);
-    launch_dilu_solve_levels<double>(m, G.g, G.dinv, dx, G.yv, dx, &G.ca, G.wrap, nullptr,
+    launch_dilu_solve_levels<double>(m, G.g, G.dinv, dx, G.yv, dx, &G.ca, G.wrap,
+                                     IluBlocks{G.g.a, G.g.b, G.g.c, G.wrap ? 1 : 0}, nullptr,
                                      G.stream);
   }
   check(hipGetLastError(), "trisolve launch");

@@ -99,6 +99,7 @@ struct p3d_solver {
   int verbose = 0;
   int guess_check = 0;
   int tile_k = 32, tile_waves = 1;
+  IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   // graphs: [type][0] FP64, [1] FP32
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gK[2] = {0, 0};
@@ -160,7 +161,7 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
       launch_dilu_sweep<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, nullptr, nullptr,
                            s->stream);
     } else {
-      launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, gd,
+      launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, s->blocks, gd,
                                   s->stream);
     }
     if (epi != EPI_NONE)
@@ -705,7 +706,8 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     if ((rc = dalloc((void**)&s->yv, N * sizeof(float))) ||
         (rc = dalloc((void**)&s->dinv, N * sizeof(float))) || (rc = dalloc((void**)&dpiv, nb)))
       return bail(rc);
-    launch_dilu_factor(s->m, s->g, dpiv, s->dinv, s->stream);
+    s->blocks = IluBlocks{s->g.a, s->g.b, s->g.c, 1};
+    launch_dilu_factor(s->m, s->g, s->blocks, dpiv, s->dinv, s->stream);
     hipStreamSynchronize(s->stream);
     hipFree(dpiv);
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
