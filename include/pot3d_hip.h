@@ -111,9 +111,16 @@ enum {
                              i.e. x0.Ax0 - 2 b.x0 >= 0 */
   P3D_OPT_TILE_K = 9,     /* planes per tile of the tile-diagonal ILU
                              schedule: 16, 32 (default) or 64 */
-  P3D_OPT_TILE_WAVES = 10 /* waves per tile of that schedule: 1 (8x8
+  P3D_OPT_TILE_WAVES = 10, /* waves per tile of that schedule: 1 (8x8
                              columns, default), 4 (16x16) or 8 (16x32, at
                              most 32 planes) */
+  P3D_OPT_ILU_BLOCKS_R = 11, /* block-Jacobi ILU0: blocks along r, theta, */
+  P3D_OPT_ILU_BLOCKS_T = 12, /* phi (default 1 each: the single-rank ILU0). */
+  P3D_OPT_ILU_BLOCKS_P = 13  /* Couplings between blocks are dropped, as in
+                                the reference's ILU0 on a grid decomposed
+                                over that many ranks (src/pot3d.F90:4940-
+                                4967); block extents are rounded up to 16,
+                                32 and 64 points. */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

@@ -100,6 +100,7 @@ struct p3d_solver {
   int guess_check = 0;
   int tile_k = 32, tile_waves = 1;
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
+  int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
   // graphs: [type][0] FP64, [1] FP32
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gK[2] = {0, 0};
@@ -156,7 +157,7 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
     const CgState* gd = guard ? s->st : nullptr;
     if (s->sweep_mode == 0 && s->sweep_ready) {
       launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
-    } else if (s->sweep_mode == 2 && s->sweep_ready && !guard) {
+    } else if (s->sweep_mode == 2 && s->sweep_ready && !guard && !s->sweep.blocked) {
       // persistent dataflow schedule: single applications only
       launch_dilu_sweep<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, nullptr, nullptr,
                            s->stream);
@@ -711,12 +712,59 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     hipStreamSynchronize(s->stream);
     hipFree(dpiv);
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
-    s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves) == 0);
+    s->sweep_ready =
+        (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks) == 0);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(P3D_EHIP, "set-up failed"));
   *out = s;
+  return P3D_OK;
+}
+
+// Block extents of the block-Jacobi cut for nblk blocks per axis, rounded
+// up to multiples of 16 (r), 32 (theta) and 64 (phi) points so that every
+// tile shape of the sweeps is aligned with them.
+static IluBlocks ilu_blocks(const p3d_solver* s) {
+  auto ext = [](int n, int nb, int q) {
+    if (nb <= 1) return n;
+    int e = (n + nb - 1) / nb;
+    e = (e + q - 1) / q * q;
+    return e >= n ? n : e;
+  };
+  IluBlocks bl;
+  bl.bi = ext(s->g.a, s->nblk[0], 16);
+  bl.bj = ext(s->g.b, s->nblk[1], 32);
+  bl.bk = ext(s->g.c, s->nblk[2], 64);
+  bl.wrap = bl.bk >= s->g.c ? 1 : 0;
+  return bl;
+}
+
+// (Re)builds the tile plan and, with refactor, the D-ILU pivots for the
+// current tile shape and block-Jacobi cut.
+static int rebuild_ilu(p3d_solver* s, bool refactor) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int t = 0; t < 2; t++) {
+    if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
+    s->gexec[t] = nullptr;
+  }
+  if (refactor) {
+    s->blocks = ilu_blocks(s);
+    double* dpiv = nullptr;
+    int rc = dalloc((void**)&dpiv, sizeof(double) * s->g.N);
+    if (rc) return rc;
+    launch_dilu_factor(s->m, s->g, s->blocks, dpiv, s->dinv, s->stream);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    hipFree(dpiv);
+    if (e != hipSuccess) return fail(P3D_EHIP, "D-ILU factorisation failed");
+  }
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int prc = build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks);
+  s->sweep_ready = prc == 0;
+  if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
+  if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");
   return P3D_OK;
 }
 
@@ -764,17 +812,16 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       if (key == P3D_OPT_TILE_K) s->tile_k = v;
       else s->tile_waves = v;
       if (s->ifprec != PREC_ILU) break;
-      HIPCHK(hipStreamSynchronize(s->stream));
-      int dev = 0, ncu = 0;
-      HIPCHK(hipGetDevice(&dev));
-      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-      s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves) == 0);
-      for (int t = 0; t < 2; t++) {
-        if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
-        s->gexec[t] = nullptr;
-      }
-      if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");
-      break;
+      return rebuild_ilu(s, false);
+    }
+    case P3D_OPT_ILU_BLOCKS_R:
+    case P3D_OPT_ILU_BLOCKS_T:
+    case P3D_OPT_ILU_BLOCKS_P: {
+      const int v = (int)value;
+      if (v < 1 || v > 1024) return fail(P3D_EINVAL, "block counts must be in 1..1024");
+      s->nblk[key - P3D_OPT_ILU_BLOCKS_R] = v;
+      if (s->ifprec != PREC_ILU) break;
+      return rebuild_ilu(s, true);
     }
     case P3D_OPT_GUESS_CHECK:
       if (value != 0 && value != 1 && value != 2)

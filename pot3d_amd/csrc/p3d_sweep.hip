@@ -503,9 +503,18 @@ __global__ void __launch_bounds__(64)
 // carries the tile's data.
 struct DiagArgs {
   int L, NI, NJ, NK;
+  const int* tiles;  // this diagonal's tiles (block-Jacobi plans), else null
+  IluBlocks bl;      // block-Jacobi cut (extents >= the grid: none)
 };
 
 __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
+  if (da.tiles) {
+    const int t = da.tiles[b];
+    I = t % da.NI;
+    J = (t / da.NI) % da.NJ;
+    Kc = t / (da.NI * da.NJ);
+    return;
+  }
   const int kc_lo = max(0, da.L - (da.NI - 1) - (da.NJ - 1));
   const int kc_hi = min(da.NK - 1, da.L);
   I = J = Kc = 0;
@@ -521,6 +530,11 @@ __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int
     }
     b -= cnt;
   }
+}
+
+static DiagArgs diag_args(const SweepPlan& plan, int L) {
+  return DiagArgs{L, plan.TNI, plan.TNJ, plan.NK,
+                  plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr, plan.bl};
 }
 
 template <int TK>
@@ -550,6 +564,10 @@ __global__ void __launch_bounds__(8 * TK)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  // couplings kept by the block-Jacobi cut (tile-aligned: inside a tile only
+  // the first plane can start a phi block)
+  const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
+  const bool wrap = da.bl.wrap != 0;
   // stage the tile: wave w loads planes 8w .. 8w+7; wave 0 also loads its
   // column factors and incoming k values in the same round trip
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
@@ -557,11 +575,11 @@ __global__ void __launch_bounds__(8 * TK)
   const double dpu_last = m.DPU[m.c - 1];
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (wave == 0) {
-    if (act && k0 > 0) {
+    if (act && lo_k0) {
       const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
       t_own = dinv[q] * yv[q];
     }
-    if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+    if (act && wrap && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
   }
   double dph_s = 0.0, dpx_s = 0.0;
   if ((int)threadIdx.x < nk) {
@@ -578,8 +596,8 @@ __global__ void __launch_bounds__(8 * TK)
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
         rv[u] = (float)r[p];
         dv[u] = dinv[p];
-        if (ii == 0 && i > 0) fiv[u] = dinv[p - 1] * yv[p - 1];
-        if (jj == 0 && j > 0) fjv[u] = dinv[p - g.a] * yv[p - g.a];
+        if (ii == 0 && lo_i) fiv[u] = dinv[p - 1] * yv[p - 1];
+        if (jj == 0 && lo_j) fjv[u] = dinv[p - g.a] * yv[p - g.a];
       }
     }
     if (done) return;  // uniform over the block
@@ -600,7 +618,7 @@ __global__ void __launch_bounds__(8 * TK)
   }
   __syncthreads();
   if (wave == 0) {
-    const float aw = (k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
+    const float aw = (wrap && k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
@@ -627,10 +645,10 @@ __global__ void __launch_bounds__(8 * TK)
         const float from_j = __shfl_up(t_pub, TI, 64);
         if (act && q >= 0 && q < nk) {
           float y = rr[u];
-          if (k == g.c - 1) y = y - aw * t_w;
-          if (k > 0) y = y - ak[u] * t_own;
-          if (j > 0) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-          if (i > 0) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+          if (wrap && k == g.c - 1) y = y - aw * t_w;
+          if (q > 0 || lo_k0) y = y - ak[u] * t_own;
+          if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
+          if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
           sh.v[q][lane] = y;
           const float tt = dd[u] * y;
           if (k == 0) t_w = tt;
@@ -666,13 +684,15 @@ __global__ void __launch_bounds__(8 * TK)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const bool hi_i = da.bl.hi_i(i, g.a), hi_j = da.bl.hi_j(j, g.b);
+  const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpl_first = m.DPL[0];
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (wave == 0) {
-    if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
-    if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+    if (act && hi_k1) t_own = yv[col + g.ab * (uint32_t)k1];
+    if (act && wrap && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
   }
   double dph_s = 0.0, dpx_s = 0.0;
   if ((int)threadIdx.x < nk) {
@@ -689,8 +709,8 @@ __global__ void __launch_bounds__(8 * TK)
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
         yv8[u] = yv[p];
         dv[u] = dinv[p];
-        if (ii == TI - 1 && i < g.a - 1) fiv[u] = yv[p + 1];
-        if (jj == TJ - 1 && j < g.b - 1) fjv[u] = yv[p + g.a];
+        if (ii == TI - 1 && hi_i) fiv[u] = yv[p + 1];
+        if (jj == TJ - 1 && hi_j) fjv[u] = yv[p + g.a];
       }
     }
     if (done) return;  // uniform over the block
@@ -711,7 +731,7 @@ __global__ void __launch_bounds__(8 * TK)
   }
   __syncthreads();
   if (wave == 0) {
-    const float bw = (k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
+    const float bw = (wrap && k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
@@ -738,10 +758,10 @@ __global__ void __launch_bounds__(8 * TK)
         const float from_j = __shfl_down(t_pub, TI, 64);
         if (act && q >= 0 && q < nk) {
           float sv = yy[u];
-          if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-          if (j < g.b - 1) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-          if (k < g.c - 1) sv = sv - bk[u] * t_own;
-          if (k == 0) sv = sv - bw * t_w;
+          if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
+          if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+          if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
+          if (wrap && k == 0) sv = sv - bw * t_w;
           const float zz = sv * dd[u];
           sh.v[q][lane] = zz;
           if (k == g.c - 1) t_w = zz;
@@ -772,12 +792,12 @@ void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          hipStream_t s) {
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK},
+    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L),
                                                  dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK},
+    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L),
                                                  dinv, yv, z, guard);
   }
 }
@@ -817,21 +837,23 @@ __global__ void __launch_bounds__(64 * SI * SJ)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
+  const bool wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpu_last = m.DPU[m.c - 1];
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (act && k0 > 0) {
+  if (act && lo_k0) {
     const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
     t_own = dinv[q] * yv[q];
   }
-  if (act && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+  if (act && wrap && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
   double dph_s = 0.0, dpx_s = 0.0;
   if (tid < nk) {
     dph_s = m.DPH[k0 + tid];
     dpx_s = m.DPL[k0 + tid];
   }
-  const bool face_i = (si == 0 && ii == 0 && i > 0), face_j = (sj == 0 && jj == 0 && j > 0);
+  const bool face_i = (si == 0 && ii == 0 && lo_i), face_j = (sj == 0 && jj == 0 && lo_j);
   for (int q0 = 0; q0 < TK; q0 += 16) {
     float rv[16], dv[16], fiv[16], fjv[16];
 #pragma unroll
@@ -863,7 +885,7 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     sh.dpx[tid] = dpx_s;
   }
   __syncthreads();
-  const float aw = (k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
+  const float aw = (wrap && k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
   const int off = 8 * (si + sj);
   const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
   for (int T0 = 0; T0 < S; T0 += 8) {
@@ -892,10 +914,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
         if (ii == 0) fl = (si == 0) ? sh.fi[q][8 * sj + jj] : sh.xi[si > 0 ? si - 1 : 0][sj][q][jj];
         if (jj == 0) fd = (sj == 0) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj > 0 ? sj - 1 : 0][q][ii];
         float y = rr[u];
-        if (k == g.c - 1) y = y - aw * t_w;
-        if (k > 0) y = y - ak[u] * t_own;
-        if (j > 0) y = y - aj[u] * fd;
-        if (i > 0) y = y - ai[u] * fl;
+        if (wrap && k == g.c - 1) y = y - aw * t_w;
+        if (q > 0 || lo_k0) y = y - ak[u] * t_own;
+        if (lo_j) y = y - aj[u] * fd;
+        if (lo_i) y = y - ai[u] * fl;
         sh.v[q][tid] = y;
         const float tt = dd[u] * y;
         if (ii == 7 && si < SI - 1) sh.xi[si][sj][q][jj] = tt;
@@ -928,19 +950,21 @@ __global__ void __launch_bounds__(64 * SI * SJ)
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
+  const bool hi_i = da.bl.hi_i(i, g.a), hi_j = da.bl.hi_j(j, g.b);
+  const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpl_first = m.DPL[0];
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  if (act && k1 < g.c) t_own = yv[col + g.ab * (uint32_t)k1];
-  if (act && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+  if (act && hi_k1) t_own = yv[col + g.ab * (uint32_t)k1];
+  if (act && wrap && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
   double dph_s = 0.0, dpx_s = 0.0;
   if (tid < nk) {
     dph_s = m.DPH[k0 + tid];
     dpx_s = m.DPU[k0 + tid];
   }
-  const bool face_i = (si == SI - 1 && ii == 7 && i < g.a - 1);
-  const bool face_j = (sj == SJ - 1 && jj == 7 && j < g.b - 1);
+  const bool face_i = (si == SI - 1 && ii == 7 && hi_i);
+  const bool face_j = (sj == SJ - 1 && jj == 7 && hi_j);
   for (int q0 = 0; q0 < TK; q0 += 16) {
     float yv16[16], dv[16], fiv[16], fjv[16];
 #pragma unroll
@@ -972,7 +996,7 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     sh.dpx[tid] = dpx_s;
   }
   __syncthreads();
-  const float bw = (k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
+  const float bw = (wrap && k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
   const int off = 8 * ((SI - 1 - si) + (SJ - 1 - sj));
   const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
   for (int T0 = 0; T0 < S; T0 += 8) {
@@ -1001,10 +1025,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
         if (ii == 7) fr = (si == SI - 1) ? sh.fi[q][8 * sj + jj] : sh.xi[si < SI - 1 ? si : 0][sj][q][jj];
         if (jj == 7) fu = (sj == SJ - 1) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj < SJ - 1 ? sj : 0][q][ii];
         float sv = yy[u];
-        if (i < g.a - 1) sv = sv - bi[u] * fr;
-        if (j < g.b - 1) sv = sv - bj[u] * fu;
-        if (k < g.c - 1) sv = sv - bk[u] * t_own;
-        if (k == 0) sv = sv - bw * t_w;
+        if (hi_i) sv = sv - bi[u] * fr;
+        if (hi_j) sv = sv - bj[u] * fu;
+        if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
+        if (wrap && k == 0) sv = sv - bw * t_w;
         const float zz = sv * dd[u];
         sh.v[q][tid] = zz;
         if (ii == 0 && si > 0) sh.xi[si - 1][sj][q][jj] = zz;
@@ -1031,12 +1055,12 @@ void tiles_launch_w(const SweepPlan& plan, const Metric& m, const Grid3& g, cons
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     k_tilew_fwd<T, SI, SJ, TK><<<cnt, 64 * SI * SJ, 0, s>>>(
-        m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK}, dinv, r, yv, guard);
+        m, g, diag_args(plan, L), dinv, r, yv, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     k_tilew_bwd<T, SI, SJ, TK><<<cnt, 64 * SI * SJ, 0, s>>>(
-        m, g, DiagArgs{L, plan.TNI, plan.TNJ, plan.NK}, dinv, yv, z, guard);
+        m, g, diag_args(plan, L), dinv, yv, z, guard);
   }
 }
 
@@ -1082,7 +1106,8 @@ void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const flo
 
 }  // namespace
 
-int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves) {
+int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves,
+                     const IluBlocks* blk) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   plan.NI = (g.a + TI - 1) / TI;
@@ -1133,16 +1158,35 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
   plan.TNI = (g.a + TI * plan.SI - 1) / (TI * plan.SI);
   plan.TNJ = (g.b + TJ * plan.SJ - 1) / (TJ * plan.SJ);
   plan.NK = (g.c + plan.TK - 1) / plan.TK;
-  plan.ndiag = plan.TNI + plan.TNJ + plan.NK - 2;
+  plan.bl = IluBlocks{g.a, g.b, g.c, 1};
+  plan.blocked = 0;
+  if (blk && (blk->bi < g.a || blk->bj < g.b || blk->bk < g.c)) {
+    const int ti = TI * plan.SI, tj = TJ * plan.SJ;
+    if ((blk->bi < g.a && blk->bi % ti) || (blk->bj < g.b && blk->bj % tj) ||
+        (blk->bk < g.c && blk->bk % plan.TK)) {
+      free_sweep_plan(plan);
+      return 2;
+    }
+    plan.bl = *blk;
+    plan.blocked = 1;
+  }
+  // tile extents of one block (the whole grid without a cut)
+  const int bIt = plan.blocked && plan.bl.bi < g.a ? plan.bl.bi / (TI * plan.SI) : plan.TNI;
+  const int bJt = plan.blocked && plan.bl.bj < g.b ? plan.bl.bj / (TJ * plan.SJ) : plan.TNJ;
+  const int bKt = plan.blocked && plan.bl.bk < g.c ? plan.bl.bk / plan.TK : plan.NK;
+  plan.ndiag = bIt + bJt + bKt - 2;
+  std::vector<std::vector<int>> by_diag(plan.ndiag);
+  for (int Kc = 0; Kc < plan.NK; Kc++)
+    for (int J = 0; J < plan.TNJ; J++)
+      for (int I = 0; I < plan.TNI; I++) {
+        const int L = (I % bIt) + (J % bJt) + (Kc % bKt);
+        by_diag[L].push_back(I + plan.TNI * (J + plan.TNJ * Kc));
+      }
   std::vector<int> tl;
   plan.diag_off.assign(plan.ndiag + 1, 0);
   for (int L = 0; L < plan.ndiag; L++) {
     plan.diag_off[L] = (int)tl.size();
-    for (int Kc = 0; Kc < plan.NK; Kc++)
-      for (int J = 0; J < plan.TNJ; J++) {
-        const int I = L - Kc - J;
-        if (I >= 0 && I < plan.TNI) tl.push_back(I + plan.TNI * (J + plan.TNJ * Kc));
-      }
+    tl.insert(tl.end(), by_diag[L].begin(), by_diag[L].end());
   }
   plan.diag_off[plan.ndiag] = (int)tl.size();
   if (hipMalloc((void**)&plan.tiles, tl.size() * sizeof(int)) != hipSuccess) {

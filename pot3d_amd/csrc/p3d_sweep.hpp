@@ -42,6 +42,10 @@ struct SweepPlan {
   // anti-diagonal; SI x SJ waves per tile)
   int TK = 0, NK = 0, ndiag = 0;
   int SI = 1, SJ = 1, TNI = 0, TNJ = 0;
+  // block-Jacobi cut: tiles are scheduled by their diagonal inside their
+  // block (the blocks are independent), tile indices come from `tiles`
+  int blocked = 0;
+  IluBlocks bl{};
   int* tiles = nullptr;            // tiles ordered by diagonal
   std::vector<int> diag_off;       // ndiag+1 offsets into tiles
 };
@@ -49,8 +53,10 @@ struct SweepPlan {
 // Returns 0 when the dataflow sweep can run on this grid.
 // tile_k: planes per tile of the tile-diagonal schedule (16, 32 or 64).
 // tile_waves: waves per tile, 1 (8x8), 4 (16x16) or 8 (16x32; tile_k <= 32).
+// bl: block-Jacobi cut of the ILU (null: none); its extents must be
+// multiples of the tile extents (returns nonzero otherwise).
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32,
-                     int tile_waves = 1);
+                     int tile_waves = 1, const IluBlocks* bl = nullptr);
 void free_sweep_plan(SweepPlan& plan);
 // 1 if a sweep wait exceeded its bound since the last call (and clears it).
 int sweep_error(const SweepPlan& plan);

@@ -18,7 +18,10 @@ module pot3d_hip
                                    P3D_OPT_GRAPH=5, P3D_OPT_SWEEP=6,    &
                                    P3D_OPT_VERBOSE=7,                   &
                                    P3D_OPT_GUESS_CHECK=8,               &
-                                   P3D_OPT_TILE_K=9, P3D_OPT_TILE_WAVES=10
+                                   P3D_OPT_TILE_K=9, P3D_OPT_TILE_WAVES=10, &
+                                   P3D_OPT_ILU_BLOCKS_R=11,             &
+                                   P3D_OPT_ILU_BLOCKS_T=12,             &
+                                   P3D_OPT_ILU_BLOCKS_P=13
 !
       interface
 ! ****** Whole-solve ABI.
