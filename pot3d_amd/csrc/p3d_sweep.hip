@@ -505,6 +505,8 @@ struct DiagArgs {
   int L, NI, NJ, NK;
   const int* tiles;  // this diagonal's tiles (block-Jacobi plans), else null
   IluBlocks bl;      // block-Jacobi cut (extents >= the grid: none)
+  float* face_i;     // per tile [TK][8]: its last (forward) / first (backward)
+  float* face_j;     //   column (face_i) and row (face_j) values, for the next tile
 };
 
 __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
@@ -533,8 +535,13 @@ __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int
 }
 
 static DiagArgs diag_args(const SweepPlan& plan, int L) {
-  return DiagArgs{L, plan.TNI, plan.TNJ, plan.NK,
-                  plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr, plan.bl};
+  return DiagArgs{L,       plan.TNI, plan.TNJ,    plan.NK,
+                  plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr,
+                  plan.bl, plan.face_i, plan.face_j};
+}
+
+__device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int Kc) {
+  return (size_t)I + (size_t)da.NI * ((size_t)J + (size_t)da.NJ * (size_t)Kc);
 }
 
 template <int TK>
@@ -547,7 +554,11 @@ struct TileLds {
 };
 
 // TK/8 waves stage and store the tile (8 planes each, one round of loads in
-// flight per wave); wave 0 marches.
+// flight per wave); wave 0 marches.  The faces a tile needs from its lower
+// (forward) or upper (backward) neighbours in i and j are not read from the
+// scattered neighbour points: every tile leaves its boundary column and row
+// in compact per-tile buffers (face_i, face_j; 8 values per plane), which
+// the next tile reads as one contiguous block per face.
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
     k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
@@ -586,18 +597,25 @@ __global__ void __launch_bounds__(8 * TK)
     dph_s = m.DPH[k0 + threadIdx.x];
     dpx_s = m.DPL[k0 + threadIdx.x];
   }
+  // incoming faces: thread -> (plane fq, index fe), one float each
+  const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
+  float fi_s = 0.f, fj_s = 0.f;
+  if (fq < nk) {
+    if (I > 0 && da.bl.lo_i(I * TI))
+      fi_s = da.face_i[(tile_id(da, I - 1, J, Kc) * TK + fq) * 8 + fe];
+    if (J > 0 && da.bl.lo_j(J * TJ))
+      fj_s = da.face_j[(tile_id(da, I, J - 1, Kc) * TK + fq) * 8 + fe];
+  }
   {
-    float rv[8], dv[8], fiv[8], fjv[8];
+    float rv[8], dv[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
-      rv[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
+      rv[u] = dv[u] = 0.f;
       if (act && q < nk) {
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
         rv[u] = (float)r[p];
         dv[u] = dinv[p];
-        if (ii == 0 && lo_i) fiv[u] = dinv[p - 1] * yv[p - 1];
-        if (jj == 0 && lo_j) fjv[u] = dinv[p - g.a] * yv[p - g.a];
       }
     }
     if (done) return;  // uniform over the block
@@ -607,10 +625,12 @@ __global__ void __launch_bounds__(8 * TK)
       if (q < nk) {
         sh.v[q][lane] = rv[u];
         sh.d[q][lane] = dv[u];
-        if (ii == 0) sh.fi[q][jj] = fiv[u];
-        if (jj == 0) sh.fj[q][ii] = fjv[u];
       }
     }
+  }
+  if (fq < nk) {
+    sh.fi[fq][fe] = fi_s;
+    sh.fj[fq][fe] = fj_s;
   }
   if ((int)threadIdx.x < nk) {
     sh.dph[threadIdx.x] = dph_s;
@@ -666,6 +686,12 @@ __global__ void __launch_bounds__(8 * TK)
       if (q < nk) yv[col + g.ab * (uint32_t)(k0 + q)] = sh.v[q][lane];
     }
   }
+  // outgoing faces t = dinv*y of the last column (ii = 7) and row (jj = 7)
+  if (fq < nk) {
+    const size_t tb = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
+    da.face_i[tb] = sh.d[fq][TI - 1 + TI * fe] * sh.v[fq][TI - 1 + TI * fe];
+    da.face_j[tb] = sh.d[fq][fe + TI * (TJ - 1)] * sh.v[fq][fe + TI * (TJ - 1)];
+  }
 }
 
 template <typename T, int TK>
@@ -699,18 +725,25 @@ __global__ void __launch_bounds__(8 * TK)
     dph_s = m.DPH[k0 + threadIdx.x];
     dpx_s = m.DPU[k0 + threadIdx.x];
   }
+  // incoming faces (z of the first column / row of the upper neighbours)
+  const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
+  float fi_s = 0.f, fj_s = 0.f;
+  if (fq < nk) {
+    if (I + 1 < da.NI && da.bl.hi_i(I * TI + TI - 1, g.a))
+      fi_s = da.face_i[(tile_id(da, I + 1, J, Kc) * TK + fq) * 8 + fe];
+    if (J + 1 < da.NJ && da.bl.hi_j(J * TJ + TJ - 1, g.b))
+      fj_s = da.face_j[(tile_id(da, I, J + 1, Kc) * TK + fq) * 8 + fe];
+  }
   {
-    float yv8[8], dv[8], fiv[8], fjv[8];
+    float yv8[8], dv[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
-      yv8[u] = dv[u] = fiv[u] = fjv[u] = 0.f;
+      yv8[u] = dv[u] = 0.f;
       if (act && q < nk) {
         const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
         yv8[u] = yv[p];
         dv[u] = dinv[p];
-        if (ii == TI - 1 && hi_i) fiv[u] = yv[p + 1];
-        if (jj == TJ - 1 && hi_j) fjv[u] = yv[p + g.a];
       }
     }
     if (done) return;  // uniform over the block
@@ -720,10 +753,12 @@ __global__ void __launch_bounds__(8 * TK)
       if (q < nk) {
         sh.v[q][lane] = yv8[u];
         sh.d[q][lane] = dv[u];
-        if (ii == TI - 1) sh.fi[q][jj] = fiv[u];
-        if (jj == TJ - 1) sh.fj[q][ii] = fjv[u];
       }
     }
+  }
+  if (fq < nk) {
+    sh.fi[fq][fe] = fi_s;
+    sh.fj[fq][fe] = fj_s;
   }
   if ((int)threadIdx.x < nk) {
     sh.dph[threadIdx.x] = dph_s;
@@ -783,6 +818,12 @@ __global__ void __launch_bounds__(8 * TK)
         z[p] = (T)zz;
       }
     }
+  }
+  // outgoing faces: z of the first column (ii = 0) and row (jj = 0)
+  if (fq < nk) {
+    const size_t tb = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
+    da.face_i[tb] = sh.v[fq][TI * fe];
+    da.face_j[tb] = sh.v[fq][fe];
   }
 }
 
@@ -1189,7 +1230,10 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     tl.insert(tl.end(), by_diag[L].begin(), by_diag[L].end());
   }
   plan.diag_off[plan.ndiag] = (int)tl.size();
-  if (hipMalloc((void**)&plan.tiles, tl.size() * sizeof(int)) != hipSuccess) {
+  const size_t nface = (size_t)plan.TNI * plan.TNJ * plan.NK * plan.TK * 8;
+  if (hipMalloc((void**)&plan.face_i, nface * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&plan.face_j, nface * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&plan.tiles, tl.size() * sizeof(int)) != hipSuccess) {
     free_sweep_plan(plan);
     return 1;
   }
@@ -1206,6 +1250,8 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.ctl) hipFree(plan.ctl);
   if (plan.item_partial) hipFree(plan.item_partial);
   if (plan.tiles) hipFree(plan.tiles);
+  if (plan.face_i) hipFree(plan.face_i);
+  if (plan.face_j) hipFree(plan.face_j);
   plan = SweepPlan();
 }
 

@@ -46,6 +46,8 @@ struct SweepPlan {
   // block (the blocks are independent), tile indices come from `tiles`
   int blocked = 0;
   IluBlocks bl{};
+  float* face_i = nullptr;  // compact tile faces of the single-wave tile sweeps
+  float* face_j = nullptr;
   int* tiles = nullptr;            // tiles ordered by diagonal
   std::vector<int> diag_off;       // ndiag+1 offsets into tiles
 };
