@@ -7,10 +7,15 @@ background), phi=0 at r=2.5, ILU0 preconditioner, stopping at
 guess with the vectors already resident in HBM.
 
 Reported:
-  value        time-to-solution of the fastest path (mixed precision: FP32
-               inner ILU0-PCG in FP64 defect correction), seconds
-  fp64_plain   the same solve with the plain FP64 ILU0 PCG (the drop-in
-               baseline), seconds, and the speed-up
+  value        time-to-solution of the fastest path, seconds: FP32 inner
+               PCG with reliable updates in FP64 defect correction, the
+               block-Jacobi ILU0 (the reference's per-rank ILU0 on a grid cut
+               into 2x8x8 blocks) and the learned warm start (network trained
+               on 51x91x181 PCG solutions, prolonged to the fine grid); the
+               timed region includes the network's prediction
+  fp64_plain   the same solve with the plain FP64 single-rank ILU0 PCG (the
+               drop-in baseline), seconds, and the speed-up; also the FP64
+               block-Jacobi and the mixed path without the network
   roofline     the dominant kernel's achieved HBM bandwidth (algorithmic
                bytes / HIP-event time on the solver's stream) vs 8 TB/s
   cpu_baseline the CPU oracle's ILU0 PCG (the reference's CPU algorithm:
@@ -157,6 +162,9 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=3)
     ap.add_argument("--ws-lr", type=float, default=1e-3)
     ap.add_argument("--ws-grid", type=str, default="51x91x181")
+    ap.add_argument("--blocks", type=str, default="2,8,8",
+                    help="block-Jacobi ILU0 blocks (r,theta,phi) of the fast paths; "
+                         "1,1,1 = single-rank ILU0")
     ap.add_argument("--guess-check", type=int, default=2,
                     help="1 residual rule, 2 A-norm rule for accepting the learned guess")
     args = ap.parse_args()
@@ -230,11 +238,21 @@ def main():
         hip.d2h(x, d_x)
         return ts, st, x
 
-    # FP64 plain ILU0 PCG (drop-in baseline)
+    # FP64 plain ILU0 PCG (drop-in baseline: the single-rank ILU0)
     t64, st64, x64 = timed("fp64 ILU0 PCG", 0)
-    # mixed precision from the zero guess
-    tmx, stmx, xmx = timed(f"mixed(mode {args.mixed_mode}) ILU0 PCG", args.mixed_mode)
     w64 = s.magnetic_energy(s.fill_phi(x64))
+    kern = measure_kernels(s, N, nr * nt * np_)
+    # the fast paths use the block-Jacobi ILU0 (the reference's per-rank ILU0
+    # on a grid decomposed into that many blocks)
+    blocks = tuple(int(v) for v in args.blocks.split(","))
+    for key, v in zip((_lib.OPT_ILU_BLOCKS_R, _lib.OPT_ILU_BLOCKS_T, _lib.OPT_ILU_BLOCKS_P),
+                      blocks):
+        s.set_option(key, v)
+    ilu_bj = {"fp64_ms": round(s.time_kernel(2, 5), 5), "fp32_ms": round(s.time_kernel(7, 5), 5)}
+    tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0)
+    # mixed precision from the zero guess
+    tmx, stmx, xmx = timed(f"mixed(mode {args.mixed_mode}) block-Jacobi ILU0 PCG",
+                           args.mixed_mode)
 
     def errs(x):
         wx = s.magnetic_energy(s.fill_phi(x))
@@ -272,9 +290,9 @@ def main():
             hip.h2d(d_phic, phic)
             W.prolong_dev(tab, d_phic, dx)
 
-        twl, stwl, xwl = timed(f"mixed(mode {args.mixed_mode}) + learned ILU0 PCG",
+        twl, stwl, xwl = timed(f"mixed(mode {args.mixed_mode}) + learned block-Jacobi ILU0 PCG",
                                args.mixed_mode, guess)
-        t64w, st64w, _ = timed("fp64 + learned ILU0 PCG", 0, guess)
+        t64w, st64w, _ = timed("fp64 + learned block-Jacobi ILU0 PCG", 0, guess)
         # quality of the guess alone
         hip.h2d(d_x, zero)
         guess(d_x)
@@ -301,7 +319,6 @@ def main():
         sc.close()
         hip.free(d_phic)
 
-    kern = measure_kernels(s, N, nr * nt * np_)
     dom = kern["dominant"]
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
@@ -323,9 +340,12 @@ def main():
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 inner", "data": "synthetic (seeded bipole+harmonic Br)",
         "config": {"workload": f"PFSS {args.grid} ILU0 PCG to ||r||2/||b||2<=1e-9",
-                   "unknowns": N, "precond": "ILU0 (D-ILU)", "epscg": 1e-9,
+                   "unknowns": N, "precond": "ILU0 (D-ILU); fast paths block-Jacobi ILU0",
+                   "epscg": 1e-9,
                    "value_path": label, "mixed_mode": args.mixed_mode},
         "fp64_plain_s": round(min(t64), 4), "fp64_plain_ncg": st64["ncg"],
+        "ilu_blocks": list(blocks), "ilu_apply_blockjacobi": ilu_bj,
+        "fp64_blockjacobi_s": round(min(tbj), 4), "fp64_blockjacobi_ncg": stbj["ncg"],
         "mixed_s": round(min(tmx), 4), "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
         "speedup_vs_fp64": round(min(t64) / best, 3),
         "mixed_errors_vs_fp64": errs(xmx),
