@@ -89,7 +89,7 @@ def cpu_baseline(workload, gpu_iters: int, sample_iters: int, threads: int):
     est = t_coef + t_setup + per_iter * gpu_iters
     return {"value": round(est, 3), "unit": "s (estimated time-to-solution)", "cores": threads,
             "kind": "port",
-            "sample": (f"oracle ILU0 PCG on the same 151x301x601 problem: matrix+ILU0 set-up "
+            "sample": (f"oracle ILU0 PCG on the same problem ({workload.name}): matrix+ILU0 set-up "
                        f"({t_coef + t_setup:.2f} s) and {sample_iters} iterations "
                        f"({per_iter:.3f} s/iteration, {threads} OpenMP threads for the "
                        f"operator, sequential trisolves), extrapolated to the GPU's "
@@ -130,19 +130,20 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
 
 def _coarse_br(args_tuple):
     """br0 of the seeded synthetic map on the coarse mesh (worker process)."""
-    mesh, seed = args_tuple
+    mesh, seed, nbip, option = args_tuple
     from pot3d_amd import inputs
-    return inputs.synthetic_br0(mesh, seed)
+    return inputs.synthetic_br0(mesh, seed, n_bipoles=nbip, option=option)
 
 
-def make_brs(mesh, seeds, workers: int):
+def make_brs(mesh, seeds, workers: int, nbip: int = 8, option: str = "ss"):
     """Boundary maps for the training set, generated on the host before the
     GPU is touched (a fork-based pool; workers=1 runs inline)."""
+    jobs = [(mesh, sd, nbip, option) for sd in seeds]
     if workers <= 1 or len(seeds) < 8:
-        return [_coarse_br((mesh, sd)) for sd in seeds]
+        return [_coarse_br(j) for j in jobs]
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(_coarse_br, [(mesh, sd) for sd in seeds], chunksize=8)
+        return pool.map(_coarse_br, jobs, chunksize=8)
 
 
 def main():
@@ -162,6 +163,9 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=3)
     ap.add_argument("--ws-lr", type=float, default=1e-3)
     ap.add_argument("--ws-grid", type=str, default="51x91x181")
+    ap.add_argument("--large", action="store_true",
+                    help="the 301x601x1201 PFCS-style problem: 'open' at r in [2.5, 10], "
+                         "16 bipoles (one step, no CPU baseline by default)")
     ap.add_argument("--blocks", type=str, default="2,8,8",
                     help="block-Jacobi ILU0 blocks (r,theta,phi) of the fast paths; "
                          "1,1,1 = single-rank ILU0")
@@ -174,18 +178,23 @@ def main():
         sys.exit(2)
 
     from pot3d_amd import workloads
+    dom = dict(option="ss", r0=1.0, r1=2.5, n_bipoles=8)
+    if args.large:
+        args.grid = "301x601x1201"
+        dom = dict(option="open", r0=2.5, r1=10.0, n_bipoles=16)
     nr, nt, np_ = (int(v) for v in args.grid.split("x"))
     t0 = time.perf_counter()
-    w = workloads.pfss(nr, nt, np_, seed=0, ifprec=2, name=f"pfss_{args.grid}")
+    w = workloads.pfss(nr, nt, np_, seed=0, ifprec=2, name=f"pfss_{args.grid}", **dom)
     print(f"[bench] workload {w.name} N={w.N} built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     ws = None
     if args.ws_pairs > 0 and not args.kernels_only:
         cnr, cnt, cnp = (int(v) for v in args.ws_grid.split("x"))
-        wc = workloads.pfss(cnr, cnt, cnp, seed=0, ifprec=2, name=f"pfss_{args.ws_grid}")
+        wc = workloads.pfss(cnr, cnt, cnp, seed=0, ifprec=2, name=f"pfss_{args.ws_grid}", **dom)
         t0 = time.perf_counter()
         seeds = list(range(1000, 1000 + args.ws_pairs))
-        brs = make_brs(wc.mesh, seeds, min(16, os.cpu_count() or 1))
+        brs = make_brs(wc.mesh, seeds, min(16, os.cpu_count() or 1), dom["n_bipoles"],
+                       dom["option"])
         ws = dict(wc=wc, brs=brs, t_maps=time.perf_counter() - t0)
         print(f"[bench] {len(brs)} training maps on {args.ws_grid} in {ws['t_maps']:.1f}s",
               file=sys.stderr, flush=True)
@@ -279,7 +288,7 @@ def main():
               file=sys.stderr, flush=True)
         # the fine problem's br0 on the coarse mesh, and the coarse padded phi
         from pot3d_amd import inputs
-        brc = inputs.synthetic_br0(wc.mesh, 0)
+        brc = inputs.synthetic_br0(wc.mesh, 0, n_bipoles=dom["n_bipoles"], option=dom["option"])
         sc.set_br0(brc)
         tab = W.prolong_tables(wc.mesh, w.mesh)
         d_phic = hip.malloc(8 * wc.mesh.nr * wc.mesh.nt * wc.mesh.np)
@@ -339,7 +348,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(tbest)), 2),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 inner", "data": "synthetic (seeded bipole+harmonic Br)",
-        "config": {"workload": f"PFSS {args.grid} ILU0 PCG to ||r||2/||b||2<=1e-9",
+        "config": {"workload": f"{'PFCS-style open' if args.large else 'PFSS'} {args.grid} "
+                               f"r in [{dom['r0']}, {dom['r1']}], {dom['n_bipoles']} bipoles, "
+                               f"ILU0 PCG to ||r||2/||b||2<=1e-9",
                    "unknowns": N, "precond": "ILU0 (D-ILU); fast paths block-Jacobi ILU0",
                    "epscg": 1e-9,
                    "value_path": label, "mixed_mode": args.mixed_mode},
