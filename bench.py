@@ -146,6 +146,20 @@ def make_brs(mesh, seeds, workers: int, nbip: int = 8, option: str = "ss"):
         return pool.map(_coarse_br, jobs, chunksize=8)
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) for this
+    workload, or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get(workload, {}).get(kernel)
+    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,7 +344,8 @@ def main():
 
     dom = kern["dominant"]
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(w.name, dom["name"]),
             "avg_ms": dom["ms"], "bytes_per_launch": dom["bytes"]}
     st_name = "stencil_fp64 (k_stencil_march<double>)"
     per_iter64 = min(t64) / max(st64["ncg"], 1)

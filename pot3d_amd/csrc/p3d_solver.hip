@@ -366,6 +366,11 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
     outer++;
     s->stats.ncg = total;
     s->stats.n_outer = outer;
+    if (s->verbose) {
+      fprintf(stderr, "[p3d] mixed: correction %d at ncg=%d, ||b-Ax||/||b|| = %.3e\n", outer,
+              total, relres);
+      fflush(stderr);
+    }
   }
   s->stats.ncg = total;
   s->stats.n_outer = outer;
@@ -441,6 +446,11 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
     relres = sqrt(rr / bb);
     s->stats.ncg = s->st_host->ncg;
     s->stats.n_outer = ++replacements;
+    if (s->verbose) {
+      fprintf(stderr, "[p3d] mixed: replacement %d at ncg=%d, ||b-Ax||/||b|| = %.3e\n",
+              replacements, s->st_host->ncg, relres);
+      fflush(stderr);
+    }
     if (!isfinite(relres)) {
       s->stats.status = 3;
       return fail(P3D_ENONFINITE, "non-finite residual in the defect correction");
