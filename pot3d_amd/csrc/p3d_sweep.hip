@@ -77,6 +77,43 @@ __device__ __forceinline__ unsigned take_ticket(unsigned* ctr) {
   return __shfl(t, 0, 64);
 }
 
+// Neighbour exchange inside one wave holding an 8x8 block of columns, lane
+// l = ii + 8 jj.  lanes_from_lower gives lane l the values of lanes l-1 and
+// l-8, lanes_from_upper those of l+1 and l+8 (meaningful where those lanes
+// are in the block).  DPP row shifts move data inside a 16-lane row; the
+// j step that crosses rows takes a rotate by 8 and the permlane16/32 swaps
+// (rows 0<->1 and 2<->3, then halves of the wave).  Everything is VALU: the
+// march's dependency chain makes no LDS round trip.
+#define P3D_DPP(v, ctrl) __builtin_amdgcn_mov_dpp((v), (ctrl), 0xf, 0xf, true)
+constexpr int kDppRowShl1 = 0x101, kDppRowShl8 = 0x108, kDppRowShr1 = 0x111,
+              kDppRowShr8 = 0x118, kDppRowRor8 = 0x128;
+
+__device__ __forceinline__ void lanes_from_lower(float t, float& from_i, float& from_j) {
+  const int v = __float_as_int(t);
+  const int lane = threadIdx.x & 63;
+  from_i = __int_as_float(P3D_DPP(v, kDppRowShr1));
+  const int sh8 = P3D_DPP(v, kDppRowShr8);      // lanes 8..15 of a row: l-8, same row
+  const int w = P3D_DPP(v, kDppRowRor8);        // lanes 0..7 of row r: lanes 8..15 of row r
+  const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
+  const auto c = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);  // {r1 x4}, {r3 x4}
+  const int row = lane >> 4;
+  const int x = (lane & 8) ? sh8 : (row == 2 ? (int)c[0] : (int)a[0]);
+  from_j = __int_as_float(x);
+}
+
+__device__ __forceinline__ void lanes_from_upper(float t, float& from_i, float& from_j) {
+  const int v = __float_as_int(t);
+  const int lane = threadIdx.x & 63;
+  from_i = __int_as_float(P3D_DPP(v, kDppRowShl1));
+  const int sh8 = P3D_DPP(v, kDppRowShl8);      // lanes 0..7 of a row: l+8, same row
+  const int w = P3D_DPP(v, kDppRowRor8);        // lanes 8..15 of row r: lanes 0..7 of row r
+  const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
+  const auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);  // {r0 x4}, {r2 x4}
+  const int row = lane >> 4;
+  const int x = (lane & 8) ? (row == 1 ? (int)c[1] : (int)a[1]) : sh8;
+  from_j = __int_as_float(x);
+}
+
 // Last wave out re-arms the ticket and progress counters for the next launch.
 __device__ void sweep_exit(unsigned* ticket, unsigned* exited, unsigned* prog, int nitems) {
   unsigned last = 0;
@@ -134,8 +171,8 @@ __device__ void fwd_item(const Metric& m, const Grid3& g, int I, int J, int item
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = s0 + u - ii - jj;
-      const float from_i = __shfl_up(t_pub, 1, 64);
-      const float from_j = __shfl_up(t_pub, TI, 64);
+      float from_i, from_j;
+      lanes_from_lower(t_pub, from_i, from_j);
       if (act && k >= 0 && k < g.c) {
         float y = rr[u];
         if (k == g.c - 1) y = y - aw * t_w;
@@ -201,8 +238,8 @@ __device__ double bwd_item(const Metric& m, const Grid3& g, int I, int J, int it
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = (g.c - 1) - (s0 + u - rii - rjj);
-      const float from_i = __shfl_down(t_pub, 1, 64);
-      const float from_j = __shfl_down(t_pub, TI, 64);
+      float from_i, from_j;
+      lanes_from_upper(t_pub, from_i, from_j);
       if (act && k >= 0 && k < g.c) {
         float sv = yy[u];
         if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
@@ -387,8 +424,8 @@ __device__ void fwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = k0 + s0 + u - ii - jj;
-      const float from_i = __shfl_up(t_pub, 1, 64);
-      const float from_j = __shfl_up(t_pub, TI, 64);
+      float from_i, from_j;
+      lanes_from_lower(t_pub, from_i, from_j);
       if (act && k >= k0 && k < k1) {
         float y = rr[u];
         if (k == g.c - 1) y = y - aw * t_w;
@@ -440,8 +477,8 @@ __device__ void bwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = (k1 - 1) - (s0 + u - rii - rjj);
-      const float from_i = __shfl_down(t_pub, 1, 64);
-      const float from_j = __shfl_down(t_pub, TI, 64);
+      float from_i, from_j;
+      lanes_from_upper(t_pub, from_i, from_j);
       if (act && k >= k0 && k < k1) {
         float sv = yy[u];
         if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
@@ -546,19 +583,26 @@ __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int 
 
 template <int TK>
 struct TileLds {
-  float v[TK][64];   // r (forward) or y (backward), overwritten by the result
-  float d[TK][64];   // dinv
-  float fi[TK][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
-  float fj[TK][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
-  double dph[TK], dpx[TK];  // DPH(k), and DPL(k) (forward) or DPU(k) (backward)
+  // per plane and lane; row TK is scratch, written by lanes whose step
+  // falls outside the tile's planes (keeps the march free of branches)
+  float v[TK + 1][64];   // r (forward) or y (backward), overwritten by the result
+  float d[TK + 1][64];   // dinv
+  float ck[TK + 1][64];  // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
+  float cj[TK + 1][64];
+  float ci[TK + 1][64];
+  float fi[TK + 1][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
+  float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
 
-// TK/8 waves stage and store the tile (8 planes each, one round of loads in
-// flight per wave); wave 0 marches.  The faces a tile needs from its lower
-// (forward) or upper (backward) neighbours in i and j are not read from the
-// scattered neighbour points: every tile leaves its boundary column and row
-// in compact per-tile buffers (face_i, face_j; 8 values per plane), which
-// the next tile reads as one contiguous block per face.
+// TK/8 waves stage the tile (8 planes each, one round of loads in flight
+// per wave) and its float couplings; wave 0 marches; all waves store.  The
+// faces a tile needs from its lower (forward) or upper (backward)
+// neighbours in i and j are not read from the scattered neighbour points:
+// every tile leaves its boundary column and row in compact per-tile
+// buffers (face_i, face_j; 8 values per plane), which the next tile reads
+// as one contiguous block per face.  The march step is branch-free: lane
+// data moves by DPP / permlane (no LDS round trip on the dependency
+// chain), and its operands for 8 steps are read from LDS in one batch.
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
     k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
@@ -579,8 +623,6 @@ __global__ void __launch_bounds__(8 * TK)
   // the first plane can start a phi block)
   const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
   const bool wrap = da.bl.wrap != 0;
-  // stage the tile: wave w loads planes 8w .. 8w+7; wave 0 also loads its
-  // column factors and incoming k values in the same round trip
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpu_last = m.DPU[m.c - 1];
@@ -591,11 +633,6 @@ __global__ void __launch_bounds__(8 * TK)
       t_own = dinv[q] * yv[q];
     }
     if (act && wrap && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
-  }
-  double dph_s = 0.0, dpx_s = 0.0;
-  if ((int)threadIdx.x < nk) {
-    dph_s = m.DPH[k0 + threadIdx.x];
-    dpx_s = m.DPL[k0 + threadIdx.x];
   }
   // incoming faces: thread -> (plane fq, index fe), one float each
   const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
@@ -623,8 +660,12 @@ __global__ void __launch_bounds__(8 * TK)
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
       if (q < nk) {
+        const double dph = m.DPH[k0 + q];
         sh.v[q][lane] = rv[u];
         sh.d[q][lane] = dv[u];
+        sh.ck[q][lane] = (float)(-(P1 * m.DPL[k0 + q]));
+        sh.cj[q][lane] = (float)(-((DRH * dph) * CTL));
+        sh.ci[q][lane] = (float)(-((SD * dph) * CRL));
       }
     }
   }
@@ -632,49 +673,42 @@ __global__ void __launch_bounds__(8 * TK)
     sh.fi[fq][fe] = fi_s;
     sh.fj[fq][fe] = fj_s;
   }
-  if ((int)threadIdx.x < nk) {
-    sh.dph[threadIdx.x] = dph_s;
-    sh.dpx[threadIdx.x] = dpx_s;
-  }
   __syncthreads();
   if (wave == 0) {
     const float aw = (wrap && k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
+    const int q_wrap = (wrap && k1 == g.c) ? nk - 1 : -1;  // plane k = c-1
+    const int q_zero = (k0 == 0) ? 0 : -1;                 // plane k = 0
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int q = s0 + u - ii - jj;
-        rr[u] = dd[u] = fl[u] = fd[u] = ak[u] = aj[u] = ai[u] = 0.f;
-        if (act && q >= 0 && q < nk) {
-          rr[u] = sh.v[q][lane];
-          dd[u] = sh.d[q][lane];
-          const double dph = sh.dph[q];
-          ak[u] = (float)(-(P1 * sh.dpx[q]));
-          aj[u] = (float)(-((DRH * dph) * CTL));
-          ai[u] = (float)(-((SD * dph) * CRL));
-          fl[u] = sh.fi[q][jj];
-          fd[u] = sh.fj[q][ii];
-        }
+        const int qs = (q >= 0 && q < nk) ? q : TK;
+        rr[u] = sh.v[qs][lane];
+        dd[u] = sh.d[qs][lane];
+        ak[u] = sh.ck[qs][lane];
+        aj[u] = sh.cj[qs][lane];
+        ai[u] = sh.ci[qs][lane];
+        fl[u] = sh.fi[qs][jj];
+        fd[u] = sh.fj[qs][ii];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int q = s0 + u - ii - jj;
-        const int k = k0 + q;
-        const float from_i = __shfl_up(t_pub, 1, 64);
-        const float from_j = __shfl_up(t_pub, TI, 64);
-        if (act && q >= 0 && q < nk) {
-          float y = rr[u];
-          if (wrap && k == g.c - 1) y = y - aw * t_w;
-          if (q > 0 || lo_k0) y = y - ak[u] * t_own;
-          if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-          if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
-          sh.v[q][lane] = y;
-          const float tt = dd[u] * y;
-          if (k == 0) t_w = tt;
-          t_own = tt;
-          t_pub = tt;
-        }
+        const bool on = act && q >= 0 && q < nk;
+        float from_i, from_j;
+        lanes_from_lower(t_pub, from_i, from_j);
+        float y = rr[u];
+        if (q == q_wrap) y = y - aw * t_w;
+        if (q > 0 || lo_k0) y = y - ak[u] * t_own;
+        if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
+        if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+        sh.v[on ? q : TK][lane] = y;
+        const float tt = dd[u] * y;
+        t_w = (on && q == q_zero) ? tt : t_w;
+        t_own = on ? tt : t_own;
+        t_pub = on ? tt : t_pub;
       }
     }
   }
@@ -720,11 +754,6 @@ __global__ void __launch_bounds__(8 * TK)
     if (act && hi_k1) t_own = yv[col + g.ab * (uint32_t)k1];
     if (act && wrap && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
   }
-  double dph_s = 0.0, dpx_s = 0.0;
-  if ((int)threadIdx.x < nk) {
-    dph_s = m.DPH[k0 + threadIdx.x];
-    dpx_s = m.DPU[k0 + threadIdx.x];
-  }
   // incoming faces (z of the first column / row of the upper neighbours)
   const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
   float fi_s = 0.f, fj_s = 0.f;
@@ -751,8 +780,12 @@ __global__ void __launch_bounds__(8 * TK)
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
       if (q < nk) {
+        const double dph = m.DPH[k0 + q];
         sh.v[q][lane] = yv8[u];
         sh.d[q][lane] = dv[u];
+        sh.ck[q][lane] = (float)(-(P1 * m.DPU[k0 + q]));
+        sh.cj[q][lane] = (float)(-((DRH * dph) * CTH));
+        sh.ci[q][lane] = (float)(-((SD * dph) * CRH));
       }
     }
   }
@@ -760,49 +793,42 @@ __global__ void __launch_bounds__(8 * TK)
     sh.fi[fq][fe] = fi_s;
     sh.fj[fq][fe] = fj_s;
   }
-  if ((int)threadIdx.x < nk) {
-    sh.dph[threadIdx.x] = dph_s;
-    sh.dpx[threadIdx.x] = dpx_s;
-  }
   __syncthreads();
   if (wave == 0) {
     const float bw = (wrap && k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
+    const int q_wrap = (wrap && k0 == 0) ? 0 : -1;  // plane k = 0
+    const int q_top = (k1 == g.c) ? nk - 1 : -1;    // plane k = c-1
     const int S = nk + SKEW;
     for (int s0 = 0; s0 < S; s0 += 8) {
       float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int q = (nk - 1) - (s0 + u - rii - rjj);
-        yy[u] = dd[u] = fr[u] = fu[u] = bk[u] = bj[u] = bi[u] = 0.f;
-        if (act && q >= 0 && q < nk) {
-          yy[u] = sh.v[q][lane];
-          dd[u] = sh.d[q][lane];
-          const double dph = sh.dph[q];
-          bk[u] = (float)(-(P1 * sh.dpx[q]));
-          bj[u] = (float)(-((DRH * dph) * CTH));
-          bi[u] = (float)(-((SD * dph) * CRH));
-          fr[u] = sh.fi[q][jj];
-          fu[u] = sh.fj[q][ii];
-        }
+        const int qs = (q >= 0 && q < nk) ? q : TK;
+        yy[u] = sh.v[qs][lane];
+        dd[u] = sh.d[qs][lane];
+        bk[u] = sh.ck[qs][lane];
+        bj[u] = sh.cj[qs][lane];
+        bi[u] = sh.ci[qs][lane];
+        fr[u] = sh.fi[qs][jj];
+        fu[u] = sh.fj[qs][ii];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int q = (nk - 1) - (s0 + u - rii - rjj);
-        const int k = k0 + q;
-        const float from_i = __shfl_down(t_pub, 1, 64);
-        const float from_j = __shfl_down(t_pub, TI, 64);
-        if (act && q >= 0 && q < nk) {
-          float sv = yy[u];
-          if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-          if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-          if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
-          if (wrap && k == 0) sv = sv - bw * t_w;
-          const float zz = sv * dd[u];
-          sh.v[q][lane] = zz;
-          if (k == g.c - 1) t_w = zz;
-          t_own = zz;
-          t_pub = zz;
-        }
+        const bool on = act && q >= 0 && q < nk;
+        float from_i, from_j;
+        lanes_from_upper(t_pub, from_i, from_j);
+        float sv = yy[u];
+        if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
+        if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+        if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
+        if (q == q_wrap) sv = sv - bw * t_w;
+        const float zz = sv * dd[u];
+        sh.v[on ? q : TK][lane] = zz;
+        t_w = (on && q == q_top) ? zz : t_w;
+        t_own = on ? zz : t_own;
+        t_pub = on ? zz : t_pub;
       }
     }
   }
@@ -948,8 +974,8 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     for (int u = 0; u < 8; u++) {
       const int q = T0 + u - off - ii - jj;
       const int k = k0 + q;
-      const float from_i = __shfl_up(t_pub, 1, 64);
-      const float from_j = __shfl_up(t_pub, 8, 64);
+      float from_i, from_j;
+      lanes_from_lower(t_pub, from_i, from_j);
       if (act && q >= 0 && q < nk) {
         float fl = from_i, fd = from_j;
         if (ii == 0) fl = (si == 0) ? sh.fi[q][8 * sj + jj] : sh.xi[si > 0 ? si - 1 : 0][sj][q][jj];
@@ -1059,8 +1085,8 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     for (int u = 0; u < 8; u++) {
       const int q = (nk - 1) - (T0 + u - off - rii - rjj);
       const int k = k0 + q;
-      const float from_i = __shfl_down(t_pub, 1, 64);
-      const float from_j = __shfl_down(t_pub, 8, 64);
+      float from_i, from_j;
+      lanes_from_upper(t_pub, from_i, from_j);
       if (act && q >= 0 && q < nk) {
         float fr = from_i, fu = from_j;
         if (ii == 7) fr = (si == SI - 1) ? sh.fi[q][8 * sj + jj] : sh.xi[si < SI - 1 ? si : 0][sj][q][jj];
