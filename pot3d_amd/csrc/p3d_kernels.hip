@@ -216,8 +216,9 @@ __global__ void __launch_bounds__(kBlock)
     const double P1 = DRH * m.CP[j];
     const bool im = i > 0, ip = i < g.a - 1, jm = j > 0, jp = j < g.b - 1;
     const T pmr = (T)m.pm_r1;
-    const T s0 = jm ? (T)0 : (T)(2.0 * pole_sum(pole_part, 0, i, g.a));
-    const T s1 = jp ? (T)0 : (T)(2.0 * pole_sum(pole_part, 1, i, g.a));
+    const double s0d = jm ? 0.0 : 2.0 * pole_sum(pole_part, 0, i, g.a);
+    const double s1d = jp ? 0.0 : 2.0 * pole_sum(pole_part, 1, i, g.a);
+    const T s0 = (T)s0d, s1 = (T)s1d;
     const uint32_t ab = g.ab, a = (uint32_t)g.a;
     const T* xc = x + mcol;
     T xkm = xc[(size_t)ab * (k0 > 0 ? k0 - 1 : g.c - 1)];
@@ -266,15 +267,27 @@ __global__ void __launch_bounds__(kBlock)
             const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
             yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
                  (T)a6 * xjp + (T)a7 * xkp;
+            if (mode == 1) yv = bv[u] - yv;
           } else {
-            // FP32: the same operator written with neighbour differences
-            // (a4 = -sum of the others).  For smooth x the two forms differ
-            // by cancellation: a4 xk + sum a_nb x_nb loses about
-            // eps32*cond(A) relative accuracy, the differences do not.
-            yv = (T)a1 * (xkm - xk) + (T)a2 * (xjm - xk) + (T)a3 * (xim - xk) +
-                 (T)a5 * (xip - xk) + (T)a6 * (xjp - xk) + (T)a7 * (xkp - xk);
+            // FP32 storage, FP64 arithmetic: the same operator written with
+            // neighbour differences (a4 = -sum of the others), formed and
+            // summed in double, only the result rounded to float.  For a
+            // smooth x, sum a_nb x_nb + a4 xk cancels to far below its terms
+            // (about 1/h^2 for a Laplacian), and so does the sum of the
+            // differences (about 1/h); in float either loses that factor
+            // times eps32, which stalls the FP32 Krylov recurrence on smooth
+            // (open-boundary, monopole-dominated) solutions.
+            // (ghost values at the boundaries in double as well)
+            const double c = (double)xk;
+            const double dim = im ? (double)xw[u] - c : 0.0;
+            const double dip = ip ? (double)xe[u] - c : (m.pm_r1 - 1.0) * c;
+            const double djm = jm ? (double)xs[u] - c : s0d - 2.0 * c;
+            const double djp = jp ? (double)xnn[u] - c : s1d - 2.0 * c;
+            double yd = a1 * ((double)xkm - c) + a2 * djm + a3 * dim + a5 * dip + a6 * djp +
+                        a7 * ((double)xkp - c);
+            if (mode == 1) yd = (double)bv[u] - yd;
+            yv = (T)yd;
           }
-          if (mode == 1) yv = bv[u] - yv;
           y[n] = yv;
           acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
           xkm = xk;
@@ -337,28 +350,40 @@ __device__ __forceinline__ T apply_point_prec(const Metric& m, const Grid3& g, i
 }
 
 // x += alpha p ; r -= alpha ap ; (DIAG/NONE) z = M^-1 r with s0=r.z, s1=r.r
-// (cgsolve :4340-4351).  For ILU only s1=r.r is reduced here.
+// (cgsolve :4340-4351).  For ILU only s1=r.r is reduced here.  With xs != nullptr (FP32 recurrence, FP64 solution) the
+// update goes to the double vector xs instead of x: xs += (alpha scale) p,
+// scale = *xscale being the factor of the scaled FP32 system.  Accumulating
+// the solution in FP32 would leave its true residual at about
+// eps32*cond(A) of the start, whatever the recurrence says.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ x,
                 T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap, CgState* st,
-                double* partial, unsigned* counter) {
+                double* partial, unsigned* counter, double* __restrict__ xs,
+                const double* __restrict__ xscale) {
   if (st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
-  const T alpha = (T)st_load(&st->alpha);
+  const double alpha_d = st_load(&st->alpha);
+  const T alpha = (T)alpha_d;
+  const double alpha_x = xs ? alpha_d * *xscale : 0.0;
   double s0 = 0.0, s1 = 0.0;
   // blocks of kUnroll points per thread, all loads issued first; the sums
   // keep the point order of the plain loop
   for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
     T xv[kUnroll], pv[kUnroll], rv[kUnroll], av[kUnroll];
+    double xd[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t n = nb + u * kBlock;
       xv[u] = pv[u] = rv[u] = av[u] = (T)0;
+      xd[u] = 0.0;
       if (n < n1) {
-        xv[u] = x[n];
+        if (xs)
+          xd[u] = xs[n];
+        else
+          xv[u] = x[n];
         pv[u] = p[n];
         rv[u] = r[n];
         av[u] = ap[n];
@@ -368,7 +393,10 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t n = nb + u * kBlock;
       if (n < n1) {
-        x[n] = xv[u] + alpha * pv[u];
+        if (xs)
+          xs[n] = xd[u] + alpha_x * (double)pv[u];
+        else
+          x[n] = xv[u] + alpha * pv[u];
         const T rn = rv[u] - alpha * av[u];
         r[n] = rn;
         if (prec != PREC_ILU) {
@@ -387,9 +415,9 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T>
 void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
-                      hipStream_t s) {
+                      hipStream_t s, double* xs, const double* xscale) {
   k_cg_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, x, r, p, ap, st, red.partial,
-                                         red.counter + slot);
+                                         red.counter + slot, xs, xscale);
 }
 
 // p = beta p + z  (:4361-4363)
@@ -553,26 +581,9 @@ __global__ void k_fill(double* p, double v, size_t n) {
   const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_fill_f(float* p, float v, size_t n) {
-  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-__global__ void k_accumulate(uint32_t N, double* __restrict__ x, float* __restrict__ c,
-                             const double* __restrict__ scale) {
-  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= N) return;
-  x[n] += *scale * (double)c[n];
-  c[n] = 0.f;
-}
-void launch_accumulate(uint32_t N, double* x, float* c, const double* scale, hipStream_t s) {
-  if (N) k_accumulate<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(N, x, c, scale);
-}
 
 void launch_fill(double* p, double v, size_t n, hipStream_t s) {
   if (n) k_fill<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, v, n);
-}
-void launch_fill_f(float* p, float v, size_t n, hipStream_t s) {
-  if (n) k_fill_f<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, v, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -981,7 +992,8 @@ void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi
                                   const T*, int, const double*, CgState*, Reduce, int, Epi,     \
                                   double*, bool, hipStream_t);                                  \
   template void launch_cg_update<T>(const Metric&, const Grid3&, const Launch&, int, T*, T*,    \
-                                    const T*, const T*, CgState*, Reduce, int, hipStream_t);    \
+                                    const T*, const T*, CgState*, Reduce, int, hipStream_t,     \
+                                    double*, const double*);                                    \
   template void launch_p_update<T>(const Metric&, const Grid3&, const Launch&, int, T*,         \
                                    const T*, const T*, CgState*, hipStream_t);                  \
   template void launch_dot2<T>(const Grid3&, const Launch&, const T*, const T*, const T*,       \

@@ -114,10 +114,12 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
 
 // CG update x += alpha p, r -= alpha ap, and for PREC_DIAG/NONE the reduction
 // of r.z and r.r with epilogue EPI_RITER; for PREC_ILU only r.r (EPI_RR).
+// With xs non-null the solution update goes to the FP64 vector xs instead:
+// xs += (alpha * *xscale) p (x is not touched).
 template <typename T>
 void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
-                      hipStream_t s);
+                      hipStream_t s, double* xs = nullptr, const double* xscale = nullptr);
 
 // p = z + beta p (z = M^-1 r computed in place for DIAG/NONE, read for ILU).
 template <typename T>
@@ -148,9 +150,6 @@ void launch_cast_scale(uint32_t N, TD* dst, const TS* src, const double* scale_n
                        const double* scale_den, int op, hipStream_t s);
 
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
-// x += scale[0] * c (FP64), then c = 0 (FP32 correction accumulated).
-void launch_accumulate(uint32_t N, double* x, float* c, const double* scale, hipStream_t s);
-void launch_fill_f(float* p, float v, size_t n, hipStream_t s);
 
 // RHS from the boundary data (closed form of the boundary trick).
 void launch_rhs(const Metric& m, const Grid3& g, const double* br0, const double* br1,

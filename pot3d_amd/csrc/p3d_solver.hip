@@ -54,6 +54,8 @@ int set_error(int code, const char* msg) {
 template <typename T>
 struct Vecs {
   T *x, *r, *p, *ap, *z;
+  double* xs = nullptr;             // FP64 solution updated instead of x (FP32 recurrence)
+  const double* xscale = nullptr;  // scale of the FP32 system (device)
 };
 
 struct p3d_solver {
@@ -101,9 +103,9 @@ struct p3d_solver {
   int tile_k = 32, tile_waves = 1;
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
-  // graphs: [type][0] FP64, [1] FP32
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  int gK[2] = {0, 0};
+  // graphs: [0] FP64, [1] FP32, [2] FP32 recurrence with the FP64 solution
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  int gK[3] = {0, 0, 0};
   // stats
   p3d_stats stats;
   int last_hist = 0;
@@ -181,7 +183,7 @@ static void cg_iteration(p3d_solver* s, Vecs<T> v) {
   launch_stencil<T>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red, 0,
                     EPI_PAP, nullptr, true, s->stream);
   launch_cg_update<T>(s->m, s->g, s->L, s->ifprec, v.x, v.r, v.p, v.ap, s->st, s->red, 2,
-                      s->stream);
+                      s->stream, v.xs, v.xscale);
   if (s->ifprec == PREC_ILU) prec_and_dot<T>(s, v.r, v.z, EPI_RZITER, nullptr, true);
   launch_p_update<T>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream);
 }
@@ -201,7 +203,7 @@ static int chunk_size(p3d_solver* s) {
 
 template <typename T>
 static int run_iterations(p3d_solver* s, Vecs<T> v, int K) {
-  const int ti = sizeof(T) == 8 ? 0 : 1;
+  const int ti = sizeof(T) == 8 ? 0 : (v.xs ? 2 : 1);
   if (s->use_graph) {
     if (!s->gexec[ti] || s->gK[ti] != K) {
       if (s->gexec[ti]) hipGraphExecDestroy(s->gexec[ti]);
@@ -380,14 +382,15 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
   return P3D_OK;
 }
 
-// FP32 inner PCG with reliable updates (residual replacement) inside the
-// FP64 defect correction.  The Krylov recurrence runs in FP32 on vectors
-// scaled by 1/||b - A x0||; whenever its residual has fallen by the factor
-// kReliableDelta since the last replacement, the FP32 correction is added to
-// the FP64 x, and the residual is replaced by the FP64 b - A x; the
-// recurrence then continues (beta from the replaced residual, p = z + beta
-// p), so the Krylov space is not restarted as in solve_mixed.  Ends when
-// ||b - A x||2/||b||2 <= epscg (true residual).
+// FP32 PCG recurrence with an FP64 solution and reliable updates (residual
+// replacement).  The Krylov recurrence (r, p, ap, z) runs in FP32 on vectors
+// scaled by 1/||b - A x0||; every solution update goes straight to the FP64
+// x (x += alpha scale p), so x does not carry FP32 rounding, whose effect on
+// b - A x grows with cond(A).  Whenever the recurrence residual has fallen
+// by the factor kReliableDelta since the last replacement, it is replaced by
+// the FP64 b - A x and the recurrence continues (beta from the replaced
+// residual, p = z + beta p): the Krylov space is not restarted as in
+// solve_mixed.  Ends when ||b - A x||2/||b||2 <= epscg (true residual).
 static constexpr double kReliableDelta = 1e-3;
 
 static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
@@ -413,9 +416,9 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
   s->out_host[5] = 1.0;
   HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, 2 * sizeof(double), hipMemcpyHostToDevice,
                         s->stream));
-  Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32};
+  // the recurrence's solution updates go straight to the FP64 x
+  Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32, s->x, s->out + 4};
   launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
-  launch_fill_f(v.x, 0.f, (size_t)s->g.N, s->stream);
   // the inner relres = ||r32|| scale / ||b|| estimates the true one
   const double final_tgt = 0.8 * epscg;
   CgState init;
@@ -440,8 +443,7 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
       s->stats.status = 3;
       return fail(P3D_ENONFINITE, "non-finite residual in the inner solve");
     }
-    // x += scale x32 ; x32 = 0 ; ap = b - A x
-    launch_accumulate(s->g.N, s->x, v.x, s->out + 4, s->stream);
+    // ap = b - A x
     if ((rc = true_residual(s, &rr, &bb))) return rc;
     relres = sqrt(rr / bb);
     s->stats.ncg = s->st_host->ncg;
@@ -547,7 +549,7 @@ const char* p3d_last_error(void) { return g_err.c_str(); }
 int p3d_destroy(p3d_solver* s) {
   if (!s) return P3D_OK;
   if (s->stream) hipStreamSynchronize(s->stream);
-  for (int t = 0; t < 2; t++)
+  for (int t = 0; t < 3; t++)
     if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
   void* bufs[] = {s->tables, s->mesh, s->br0, s->br1, s->x, s->r, s->p, s->ap, s->z, s->b,
                   s->yv, s->dinv, s->x32, s->r32, s->p32, s->ap32, s->z32, s->pole_part,
@@ -754,7 +756,7 @@ static IluBlocks ilu_blocks(const p3d_solver* s) {
 // current tile shape and block-Jacobi cut.
 static int rebuild_ilu(p3d_solver* s, bool refactor) {
   HIPCHK(hipStreamSynchronize(s->stream));
-  for (int t = 0; t < 2; t++) {
+  for (int t = 0; t < 3; t++) {
     if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
     s->gexec[t] = nullptr;
   }
@@ -804,7 +806,7 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       if (value != 0 && value != 1 && value != 2)
         return fail(P3D_EINVAL, "P3D_OPT_SWEEP must be 0, 1 or 2");
       s->sweep_mode = (int)value;
-      for (int t = 0; t < 2; t++) {
+      for (int t = 0; t < 3; t++) {
         if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
         s->gexec[t] = nullptr;
       }

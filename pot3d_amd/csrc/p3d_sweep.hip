@@ -15,6 +15,7 @@
 // agent-scope loads and, once satisfied, issues an agent-scope acquire
 // fence (stale lines in its XCD's L2 are dropped) before reading faces.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "p3d_kernels.hpp"
@@ -675,42 +676,59 @@ __global__ void __launch_bounds__(8 * TK)
   }
   __syncthreads();
   if (wave == 0) {
-    const float aw = (wrap && k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
-    const int q_wrap = (wrap && k1 == g.c) ? nk - 1 : -1;  // plane k = c-1
-    const int q_zero = (k0 == 0) ? 0 : -1;                 // plane k = 0
+    // Lane (ii, jj) handles plane q = s - ii - jj at step s.  t_k carries
+    // the lane's latest t = dinv y (the k-1 coupling of its next plane and
+    // what its i+1 / j+1 neighbours read one step later); it is refreshed
+    // on every step, valid or not, because a neighbour only reads it on
+    // steps where both planes are inside the tile.  The plane below the
+    // tile enters at q == 0 as t_init.  Only tiles holding plane c-1 of a
+    // wrapped phi range take the seam coupling (to plane 0).
+    const bool wrap_tile = wrap && k1 == g.c;
+    const float aw = wrap_tile ? (float)(-(P1 * dpu_last)) : 0.f;
+    const float t_init = t_own;
     const int S = nk + SKEW;
-    for (int s0 = 0; s0 < S; s0 += 8) {
-      float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
+    auto march = [&](auto seam) {
+      constexpr bool W = decltype(seam)::value;
+      float t_k = t_init, tw = t_w;
+      for (int s0 = 0; s0 < S; s0 += 8) {
+        float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int q = s0 + u - ii - jj;
-        const int qs = (q >= 0 && q < nk) ? q : TK;
-        rr[u] = sh.v[qs][lane];
-        dd[u] = sh.d[qs][lane];
-        ak[u] = sh.ck[qs][lane];
-        aj[u] = sh.cj[qs][lane];
-        ai[u] = sh.ci[qs][lane];
-        fl[u] = sh.fi[qs][jj];
-        fd[u] = sh.fj[qs][ii];
-      }
+        for (int u = 0; u < 8; u++) {
+          const int qs = min(max(s0 + u - ii - jj, 0), nk - 1);
+          rr[u] = sh.v[qs][lane];
+          dd[u] = sh.d[qs][lane];
+          ak[u] = sh.ck[qs][lane];
+          aj[u] = sh.cj[qs][lane];
+          ai[u] = sh.ci[qs][lane];
+          fl[u] = sh.fi[qs][jj];
+          fd[u] = sh.fj[qs][ii];
+        }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int q = s0 + u - ii - jj;
-        const bool on = act && q >= 0 && q < nk;
-        float from_i, from_j;
-        lanes_from_lower(t_pub, from_i, from_j);
-        float y = rr[u];
-        if (q == q_wrap) y = y - aw * t_w;
-        if (q > 0 || lo_k0) y = y - ak[u] * t_own;
-        if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-        if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
-        sh.v[on ? q : TK][lane] = y;
-        const float tt = dd[u] * y;
-        t_w = (on && q == q_zero) ? tt : t_w;
-        t_own = on ? tt : t_own;
-        t_pub = on ? tt : t_pub;
+        for (int u = 0; u < 8; u++) {
+          const int q = s0 + u - ii - jj;
+          float from_i, from_j;
+          lanes_from_lower(t_k, from_i, from_j);
+          float y = rr[u];
+          if constexpr (W) {
+            if (q == nk - 1) y = y - aw * tw;
+          }
+          const float tk = (q > 0) ? t_k : t_init;
+          if (q > 0 || lo_k0) y = y - ak[u] * tk;
+          if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
+          if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+          sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = y;
+          const float tt = dd[u] * y;
+          if constexpr (W) {
+            if (k0 == 0 && q == 0) tw = tt;
+          }
+          t_k = tt;
+        }
       }
-    }
+    };
+    if (wrap_tile)
+      march(std::true_type{});
+    else
+      march(std::false_type{});
   }
   __syncthreads();
   if (act) {
@@ -795,42 +813,55 @@ __global__ void __launch_bounds__(8 * TK)
   }
   __syncthreads();
   if (wave == 0) {
-    const float bw = (wrap && k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
-    const int q_wrap = (wrap && k0 == 0) ? 0 : -1;  // plane k = 0
-    const int q_top = (k1 == g.c) ? nk - 1 : -1;    // plane k = c-1
+    // as in the forward sweep, mirrored: q = (nk-1) - (s - rii - rjj), the
+    // plane above the tile enters at q == nk-1, and only tiles holding
+    // plane 0 of a wrapped phi range take the seam coupling (to c-1)
+    const bool wrap_tile = wrap && k0 == 0;
+    const float bw = wrap_tile ? (float)(-(P1 * dpl_first)) : 0.f;
+    const float t_init = t_own;
     const int S = nk + SKEW;
-    for (int s0 = 0; s0 < S; s0 += 8) {
-      float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
+    auto march = [&](auto seam) {
+      constexpr bool W = decltype(seam)::value;
+      float t_k = t_init, tw = t_w;
+      for (int s0 = 0; s0 < S; s0 += 8) {
+        float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int q = (nk - 1) - (s0 + u - rii - rjj);
-        const int qs = (q >= 0 && q < nk) ? q : TK;
-        yy[u] = sh.v[qs][lane];
-        dd[u] = sh.d[qs][lane];
-        bk[u] = sh.ck[qs][lane];
-        bj[u] = sh.cj[qs][lane];
-        bi[u] = sh.ci[qs][lane];
-        fr[u] = sh.fi[qs][jj];
-        fu[u] = sh.fj[qs][ii];
-      }
+        for (int u = 0; u < 8; u++) {
+          const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), 0), nk - 1);
+          yy[u] = sh.v[qs][lane];
+          dd[u] = sh.d[qs][lane];
+          bk[u] = sh.ck[qs][lane];
+          bj[u] = sh.cj[qs][lane];
+          bi[u] = sh.ci[qs][lane];
+          fr[u] = sh.fi[qs][jj];
+          fu[u] = sh.fj[qs][ii];
+        }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int q = (nk - 1) - (s0 + u - rii - rjj);
-        const bool on = act && q >= 0 && q < nk;
-        float from_i, from_j;
-        lanes_from_upper(t_pub, from_i, from_j);
-        float sv = yy[u];
-        if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-        if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-        if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
-        if (q == q_wrap) sv = sv - bw * t_w;
-        const float zz = sv * dd[u];
-        sh.v[on ? q : TK][lane] = zz;
-        t_w = (on && q == q_top) ? zz : t_w;
-        t_own = on ? zz : t_own;
-        t_pub = on ? zz : t_pub;
+        for (int u = 0; u < 8; u++) {
+          const int q = (nk - 1) - (s0 + u - rii - rjj);
+          float from_i, from_j;
+          lanes_from_upper(t_k, from_i, from_j);
+          float sv = yy[u];
+          if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
+          if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+          const float tk = (q < nk - 1) ? t_k : t_init;
+          if (q < nk - 1 || hi_k1) sv = sv - bk[u] * tk;
+          if constexpr (W) {
+            if (q == 0) sv = sv - bw * tw;
+          }
+          const float zz = sv * dd[u];
+          sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = zz;
+          if constexpr (W) {
+            if (k1 == g.c && q == nk - 1) tw = zz;
+          }
+          t_k = zz;
+        }
       }
-    }
+    };
+    if (wrap_tile)
+      march(std::true_type{});
+    else
+      march(std::false_type{});
   }
   __syncthreads();
   if (act) {
