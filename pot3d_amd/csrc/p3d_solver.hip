@@ -724,8 +724,8 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     hipStreamSynchronize(s->stream);
     hipFree(dpiv);
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
-    s->sweep_ready =
-        (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks) == 0);
+    s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
+                                       &s->blocks, s->dinv) == 0);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -773,7 +773,8 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
   int dev = 0, ncu = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int prc = build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks);
+  const int prc =
+      build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv);
   s->sweep_ready = prc == 0;
   if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
   if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");

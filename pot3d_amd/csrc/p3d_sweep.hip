@@ -545,6 +545,8 @@ struct DiagArgs {
   IluBlocks bl;      // block-Jacobi cut (extents >= the grid: none)
   float* face_i;     // per tile [TK][8]: its last (forward) / first (backward)
   float* face_j;     //   column (face_i) and row (face_j) values, for the next tile
+  const float* dinv_t;  // dinv, tile-major (tile, plane, lane); zero outside the grid
+  float* yv_t;          // forward result y / backward z in FP32, tile-major
 };
 
 __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
@@ -575,7 +577,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int
 static DiagArgs diag_args(const SweepPlan& plan, int L) {
   return DiagArgs{L,       plan.TNI, plan.TNJ,    plan.NK,
                   plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr,
-                  plan.bl, plan.face_i, plan.face_j};
+                  plan.bl, plan.face_i, plan.face_j, plan.dinv_t, plan.yv_t};
 }
 
 __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int Kc) {
@@ -606,8 +608,8 @@ struct TileLds {
 // chain), and its operands for 8 steps are read from LDS in one batch.
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
-    k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
-                   const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
+    k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const T* __restrict__ r,
+                   const CgState* guard) {
   // the done flag is read with the tile's data and tested before any store
   const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK> sh;
@@ -627,13 +629,19 @@ __global__ void __launch_bounds__(8 * TK)
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpu_last = m.DPU[m.c - 1];
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  // this tile's block of the tile-major FP32 arrays (plane q at tb + 64 q)
+  constexpr size_t kTile = (size_t)TK * 64;
+  const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
+  float t_own = 0.f, t_w = 0.f;
   if (wave == 0) {
-    if (act && lo_k0) {
-      const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
-      t_own = dinv[q] * yv[q];
+    if (act && lo_k0) {  // plane k0-1: the last plane of the tile below
+      const size_t q = tile_id(da, I, J, Kc - 1) * kTile + (TK - 1) * 64 + lane;
+      t_own = da.dinv_t[q] * da.yv_t[q];
     }
-    if (act && wrap && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
+    if (act && wrap && k0 > 0 && k1 == g.c) {  // plane 0
+      const size_t q = tile_id(da, I, J, 0) * kTile + lane;
+      t_w = da.dinv_t[q] * da.yv_t[q];
+    }
   }
   // incoming faces: thread -> (plane fq, index fe), one float each
   const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
@@ -650,10 +658,9 @@ __global__ void __launch_bounds__(8 * TK)
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
       rv[u] = dv[u] = 0.f;
-      if (act && q < nk) {
-        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
-        rv[u] = (float)r[p];
-        dv[u] = dinv[p];
+      if (q < nk) {
+        dv[u] = da.dinv_t[tb + (size_t)q * 64];
+        if (act) rv[u] = (float)r[col + g.ab * (uint32_t)(k0 + q)];
       }
     }
     if (done) return;  // uniform over the block
@@ -735,7 +742,7 @@ __global__ void __launch_bounds__(8 * TK)
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
-      if (q < nk) yv[col + g.ab * (uint32_t)(k0 + q)] = sh.v[q][lane];
+      if (q < nk) da.yv_t[tb + (size_t)q * 64] = sh.v[q][lane];
     }
   }
   // outgoing faces t = dinv*y of the last column (ii = 7) and row (jj = 7)
@@ -748,8 +755,7 @@ __global__ void __launch_bounds__(8 * TK)
 
 template <typename T, int TK>
 __global__ void __launch_bounds__(8 * TK)
-    k_tile_bwd_lds(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ dinv,
-                   float* __restrict__ yv, T* __restrict__ z, const CgState* guard) {
+    k_tile_bwd_lds(Metric m, Grid3 g, DiagArgs da, T* __restrict__ z, const CgState* guard) {
   const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK> sh;
   int I, J, Kc;
@@ -767,10 +773,15 @@ __global__ void __launch_bounds__(8 * TK)
   const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpl_first = m.DPL[0];
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
+  constexpr size_t kTile = (size_t)TK * 64;
+  const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
+  float t_own = 0.f, t_w = 0.f;
   if (wave == 0) {
-    if (act && hi_k1) t_own = yv[col + g.ab * (uint32_t)k1];
-    if (act && wrap && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
+    // plane k1: the first plane of the tile above; plane c-1: in the last tile
+    if (act && hi_k1) t_own = da.yv_t[tile_id(da, I, J, Kc + 1) * kTile + lane];
+    if (act && wrap && k1 < g.c && k0 == 0)
+      t_w = da.yv_t[tile_id(da, I, J, da.NK - 1) * kTile +
+                    (size_t)((g.c - 1) - (da.NK - 1) * TK) * 64 + lane];
   }
   // incoming faces (z of the first column / row of the upper neighbours)
   const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
@@ -788,9 +799,8 @@ __global__ void __launch_bounds__(8 * TK)
       const int q = 8 * wave + u;
       yv8[u] = dv[u] = 0.f;
       if (act && q < nk) {
-        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
-        yv8[u] = yv[p];
-        dv[u] = dinv[p];
+        yv8[u] = da.yv_t[tb + (size_t)q * 64];
+        dv[u] = da.dinv_t[tb + (size_t)q * 64];
       }
     }
     if (done) return;  // uniform over the block
@@ -869,10 +879,9 @@ __global__ void __launch_bounds__(8 * TK)
     for (int u = 0; u < 8; u++) {
       const int q = 8 * wave + u;
       if (q < nk) {
-        const uint32_t p = col + g.ab * (uint32_t)(k0 + q);
         const float zz = sh.v[q][lane];
-        yv[p] = zz;
-        z[p] = (T)zz;
+        da.yv_t[tb + (size_t)q * 64] = zz;
+        z[col + g.ab * (uint32_t)(k0 + q)] = (T)zz;
       }
     }
   }
@@ -888,15 +897,15 @@ template <typename T, int TK>
 void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          const float* dinv, const T* r, float* yv, T* z, const CgState* guard,
                          hipStream_t s) {
+  (void)dinv;
+  (void)yv;  // the single-wave sweeps use the plan's tile-major copies
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L),
-                                                 dinv, r, yv, guard);
+    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L), r, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L),
-                                                 dinv, yv, z, guard);
+    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L), z, guard);
   }
 }
 
@@ -1204,8 +1213,28 @@ void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const flo
 
 }  // namespace
 
+// dinv (natural order) -> tile-major copy: element t of tile (I, J, Kc),
+// plane q, lane ii + 8 jj holds point (8I + ii, 8J + jj, TK Kc + q); zero
+// where that point is outside the grid.
+__global__ void k_tile_gather(Grid3 g, int TNI, int TNJ, int TK, size_t n,
+                              const float* __restrict__ src, float* __restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int lane = (int)(t & 63);
+  const size_t pq = t >> 6;  // tile * TK + q
+  const int q = (int)(pq % (size_t)TK);
+  const size_t tile = pq / (size_t)TK;
+  const int I = (int)(tile % (size_t)TNI);
+  const int J = (int)((tile / (size_t)TNI) % (size_t)TNJ);
+  const int Kc = (int)(tile / ((size_t)TNI * TNJ));
+  const int i = I * TI + (lane & 7), j = J * TJ + (lane >> 3), k = Kc * TK + q;
+  dst[t] = (i < g.a && j < g.b && k < g.c)
+               ? src[(size_t)i + (size_t)g.a * j + (size_t)g.ab * k]
+               : 0.f;
+}
+
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves,
-                     const IluBlocks* blk) {
+                     const IluBlocks* blk, const float* dinv) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   plan.NI = (g.a + TI - 1) / TI;
@@ -1295,6 +1324,26 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     return 1;
   }
   hipMemcpy(plan.tiles, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (plan.SI == 1 && plan.SJ == 1) {
+    // tile-major dinv and y of the single-wave sweeps: every tile reads and
+    // writes whole 256-byte planes instead of 32-byte row pieces
+    plan.tile_elems = (size_t)plan.TNI * plan.TNJ * plan.NK * plan.TK * 64;
+    if (hipMalloc((void**)&plan.dinv_t, plan.tile_elems * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&plan.yv_t, plan.tile_elems * sizeof(float)) != hipSuccess) {
+      free_sweep_plan(plan);
+      return 1;
+    }
+    if (dinv) {
+      k_tile_gather<<<(unsigned)((plan.tile_elems + 255) / 256), 256>>>(
+          g, plan.TNI, plan.TNJ, plan.TK, plan.tile_elems, dinv, plan.dinv_t);
+    } else {
+      hipMemset(plan.dinv_t, 0, plan.tile_elems * sizeof(float));
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+      free_sweep_plan(plan);
+      return 1;
+    }
+  }
   plan.ready = 1;
   return 0;
 }
@@ -1309,6 +1358,8 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.tiles) hipFree(plan.tiles);
   if (plan.face_i) hipFree(plan.face_i);
   if (plan.face_j) hipFree(plan.face_j);
+  if (plan.dinv_t) hipFree(plan.dinv_t);
+  if (plan.yv_t) hipFree(plan.yv_t);
   plan = SweepPlan();
 }
 

@@ -50,6 +50,11 @@ struct SweepPlan {
   float* face_j = nullptr;
   int* tiles = nullptr;            // tiles ordered by diagonal
   std::vector<int> diag_off;       // ndiag+1 offsets into tiles
+  // single-wave sweeps: dinv and the FP32 sweep vector in tile-major order
+  // (tile I + TNI (J + TNJ Kc), then plane, then lane ii + 8 jj)
+  size_t tile_elems = 0;
+  float* dinv_t = nullptr;
+  float* yv_t = nullptr;
 };
 
 // Returns 0 when the dataflow sweep can run on this grid.
@@ -57,8 +62,11 @@ struct SweepPlan {
 // tile_waves: waves per tile, 1 (8x8), 4 (16x16) or 8 (16x32; tile_k <= 32).
 // bl: block-Jacobi cut of the ILU (null: none); its extents must be
 // multiples of the tile extents (returns nonzero otherwise).
+// dinv: the D-ILU's inverse pivots (natural order), copied tile-major for
+// the single-wave tile sweeps (build the plan again after refactoring).
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32,
-                     int tile_waves = 1, const IluBlocks* bl = nullptr);
+                     int tile_waves = 1, const IluBlocks* bl = nullptr,
+                     const float* dinv = nullptr);
 void free_sweep_plan(SweepPlan& plan);
 // 1 if a sweep wait exceeded its bound since the last call (and clears it).
 int sweep_error(const SweepPlan& plan);

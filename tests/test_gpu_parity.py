@@ -281,6 +281,37 @@ def test_two_norm_stop_and_mixed_precision(oracle_mod, solver_mod, mixed):
             assert np.allclose(s.magnetic_energy(), wref, rtol=1e-7, atol=0)
 
 
+def test_mixed_reliable_updates_open_boundary(solver_mod):
+    """The FP32 recurrence with the FP64 solution (mode 2) on the 'open'
+    problem (r in [2.5, 10], |Br| monopole: smooth, badly conditioned):
+    its true residual follows the recurrence, so it needs about the FP64
+    iteration count and three residual replacements for 1e-9, and lands on
+    the FP64 potential and energy."""
+    say("[gpu] mixed mode 2 on the open problem")
+    from pot3d_amd import _lib, workloads
+    w = workloads.pfss(41, 81, 160, seed=3, n_bipoles=16, option="open", r0=2.5, r1=10.0,
+                       ifprec=2)
+    with solver_mod.Pot3dSolver(w.mesh, w.pm_r1, 2) as s:
+        s.set_br0(w.br0)
+        s.set_option(_lib.OPT_STOP, 1)
+        x64 = np.zeros(s.N)
+        assert s.solve(x64, None, 1e-9, ncgmax=5000)[0] == 0
+        n64 = s.stats()["ncg"]
+        s.fill_phi()
+        w64 = s.magnetic_energy()
+        s.set_option(_lib.OPT_MIXED, 2)
+        x = np.zeros(s.N)
+        assert s.solve(x, None, 1e-9, ncgmax=5000)[0] == 0
+        st = s.stats()
+        say(f"  fp64 ncg={n64}; mixed ncg={st['ncg']} replacements={st['n_outer']} "
+            f"relres={st['true_relres']:.3e}")
+        assert st["true_relres"] <= 1e-9
+        assert st["ncg"] <= 1.05 * n64 + 10 and st["n_outer"] <= 3
+        assert _rel(x, x64) < 1e-8
+        s.fill_phi()
+        assert np.allclose(s.magnetic_energy(), w64, rtol=1e-7, atol=0)
+
+
 def test_failure_paths(oracle_mod, solver_mod):
     say("[gpu] failure paths")
     mesh = small_mesh()
