@@ -115,6 +115,30 @@ __device__ __forceinline__ void unflatten(const Grid3& g, uint32_t n, int& i, in
   k = (int)kk;
 }
 
+// Placement of the vectors the single-wave ILU tile sweeps read and write
+// (the CG residual r and preconditioned residual z): with on != 0, point
+// (i, j, k) sits at ((tile << tk_log2) + q) * 64 + 8 jj + ii, tile = I + TNI
+// (J + TNJ Kc), for I = i/8, ii = i%8, J = j/8, jj = j%8, Kc = k/TK,
+// q = k%TK (TK = 1 << tk_log2): every 8x8xTK tile of the sweeps is one
+// contiguous block of whole 64-point planes.  Entries of the block that are
+// outside the grid stay zero.  on == 0: natural order.
+struct TileMap {
+  int on = 0;
+  int TNI = 0, TNJ = 0;
+  int tk_log2 = 0;
+};
+
+__device__ __forceinline__ uint32_t tmap(const TileMap& tm, const Grid3& g, uint32_t n) {
+  if (!tm.on) return n;
+  int i, j, k;
+  unflatten(g, n, i, j, k);
+  const uint32_t tile =
+      (uint32_t)(i >> 3) +
+      (uint32_t)tm.TNI * ((uint32_t)(j >> 3) + (uint32_t)tm.TNJ * ((uint32_t)k >> tm.tk_log2));
+  const uint32_t q = (uint32_t)k & ((1u << tm.tk_log2) - 1u);
+  return (((tile << tm.tk_log2) + q) << 6) + (uint32_t)(((j & 7) << 3) | (i & 7));
+}
+
 // Device-side solver state (one per solve, in device memory).
 struct CgState {
   double bdotb;      // b . M^-1 b

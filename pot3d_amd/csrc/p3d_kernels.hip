@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(kBlock)
     k_cg_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ x,
                 T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap, CgState* st,
                 double* partial, unsigned* counter, double* __restrict__ xs,
-                const double* __restrict__ xscale) {
+                const double* __restrict__ xscale, TileMap tm) {
   if (st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
@@ -374,18 +374,21 @@ __global__ void __launch_bounds__(kBlock)
   for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
     T xv[kUnroll], pv[kUnroll], rv[kUnroll], av[kUnroll];
     double xd[kUnroll];
+    uint32_t tr[kUnroll];  // position of point n in r
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t n = nb + u * kBlock;
       xv[u] = pv[u] = rv[u] = av[u] = (T)0;
       xd[u] = 0.0;
+      tr[u] = 0;
       if (n < n1) {
+        tr[u] = tmap(tm, g, n);
         if (xs)
           xd[u] = xs[n];
         else
           xv[u] = x[n];
         pv[u] = p[n];
-        rv[u] = r[n];
+        rv[u] = r[tr[u]];
         av[u] = ap[n];
       }
     }
@@ -398,7 +401,7 @@ __global__ void __launch_bounds__(kBlock)
         else
           x[n] = xv[u] + alpha * pv[u];
         const T rn = rv[u] - alpha * av[u];
-        r[n] = rn;
+        r[tr[u]] = rn;
         if (prec != PREC_ILU) {
           const T z = apply_point_prec<T>(m, g, prec, n, rn);
           s0 += (double)rn * (double)z;
@@ -415,16 +418,16 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T>
 void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
-                      hipStream_t s, double* xs, const double* xscale) {
+                      hipStream_t s, double* xs, const double* xscale, const TileMap& tm) {
   k_cg_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, x, r, p, ap, st, red.partial,
-                                         red.counter + slot, xs, xscale);
+                                         red.counter + slot, xs, xscale, tm);
 }
 
 // p = beta p + z  (:4361-4363)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_p_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ p,
-               const T* __restrict__ r, const T* __restrict__ z, CgState* st) {
+               const T* __restrict__ r, const T* __restrict__ z, CgState* st, TileMap tm) {
   if (st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
@@ -438,7 +441,7 @@ __global__ void __launch_bounds__(kBlock)
       pv[u] = zv[u] = (T)0;
       if (n < n1) {
         pv[u] = p[n];
-        zv[u] = (prec == PREC_ILU) ? z[n] : r[n];
+        zv[u] = (prec == PREC_ILU) ? z[tmap(tm, g, n)] : r[n];
       }
     }
 #pragma unroll
@@ -454,8 +457,8 @@ __global__ void __launch_bounds__(kBlock)
 
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
-                     const T* r, const T* z, CgState* st, hipStream_t s) {
-  k_p_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, p, r, z, st);
+                     const T* r, const T* z, CgState* st, hipStream_t s, const TileMap& tm) {
+  k_p_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, p, r, z, st, tm);
 }
 
 // s0 = x.y, s1 = u.v
@@ -463,7 +466,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_dot2(Grid3 g, int G, int CH, const T* __restrict__ x, const T* __restrict__ y,
            const T* __restrict__ u, const T* __restrict__ v, CgState* st, double* partial,
-           unsigned* counter, Epi epi, double* out, int guard) {
+           unsigned* counter, Epi epi, double* out, int guard, TileMap tm) {
   if (guard && st_done(st)) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
@@ -476,11 +479,12 @@ __global__ void __launch_bounds__(kBlock)
       const uint32_t n = nb + q * kBlock;
       xv[q] = yv[q] = uv[q] = vv[q] = (T)0;
       if (n < n1) {
-        xv[q] = x[n];
-        yv[q] = y[n];
+        const uint32_t t = tmap(tm, g, n);
+        xv[q] = x[t];
+        yv[q] = y[t];
         if (u) {
-          uv[q] = u[n];
-          vv[q] = v[n];
+          uv[q] = u[t];
+          vv[q] = v[t];
         }
       }
     }
@@ -500,9 +504,9 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T>
 void launch_dot2(const Grid3& g, const Launch& L, const T* x, const T* y, const T* u,
                  const T* v, CgState* st, Reduce red, int slot, Epi epi, double* out,
-                 bool guard, hipStream_t s) {
+                 bool guard, hipStream_t s, const TileMap& tm) {
   k_dot2<T><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, x, y, u, v, st, red.partial,
-                                    red.counter + slot, epi, out, guard ? 1 : 0);
+                                    red.counter + slot, epi, out, guard ? 1 : 0, tm);
 }
 
 // z = M^-1 r (DIAG/NONE), reductions s0=r.z, s1=r.r.
@@ -536,45 +540,58 @@ void launch_point_prec(const Metric& m, const Grid3& g, const Launch& L, int pre
 }
 
 template <typename T>
-__global__ void k_sub(uint32_t N, T* __restrict__ r, const T* __restrict__ ap) {
+__global__ void k_sub(Grid3 g, T* __restrict__ r, const T* __restrict__ ap, TileMap tm) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (n < N) r[n] = r[n] - ap[n];
+  if (n >= g.N) return;
+  const uint32_t t = tmap(tm, g, n);
+  r[t] = r[t] - ap[n];
 }
 
 template <typename T>
-void launch_sub(const Grid3& g, const Launch&, T* r, const T* ap, hipStream_t s) {
-  k_sub<T><<<(g.N + kBlock - 1) / kBlock, kBlock, 0, s>>>(g.N, r, ap);
+void launch_sub(const Grid3& g, const Launch&, T* r, const T* ap, hipStream_t s,
+                const TileMap& tm) {
+  k_sub<T><<<(g.N + kBlock - 1) / kBlock, kBlock, 0, s>>>(g, r, ap, tm);
 }
 
+// dst[n] = src[n], or with a tile map: gather dst[n] = src[tmap(n)]
+// (scatter == 0) or scatter dst[tmap(n)] = src[n] (scatter == 1).
 template <typename T>
-__global__ void k_copy(uint32_t N, T* __restrict__ d, const T* __restrict__ src) {
+__global__ void k_copy(Grid3 g, T* __restrict__ d, const T* __restrict__ src, TileMap tm,
+                       int scatter) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (n < N) d[n] = src[n];
+  if (n >= g.N) return;
+  const uint32_t t = tmap(tm, g, n);
+  if (scatter)
+    d[t] = src[n];
+  else
+    d[n] = src[t];
 }
 
 template <typename T>
-void launch_copy(const Grid3& g, const Launch&, T* dst, const T* src, hipStream_t s) {
-  k_copy<T><<<(g.N + kBlock - 1) / kBlock, kBlock, 0, s>>>(g.N, dst, src);
+void launch_copy(const Grid3& g, const Launch&, T* dst, const T* src, hipStream_t s,
+                 const TileMap& tm, int scatter) {
+  k_copy<T><<<(g.N + kBlock - 1) / kBlock, kBlock, 0, s>>>(g, dst, src, tm, scatter);
 }
 
-// op 0: dst = (TD)(src * num/den)      op 1: dst += (TD)src * den/num
+// op 0: dst[tmap(n)] = (TD)(src[n] * num/den)      op 1: dst[n] += (TD)src[n] * den/num
 template <typename TD, typename TS>
-__global__ void k_cast_scale(uint32_t N, TD* __restrict__ d, const TS* __restrict__ src,
-                             const double* num, const double* den, int op) {
+__global__ void k_cast_scale(Grid3 g, TD* __restrict__ d, const TS* __restrict__ src,
+                             const double* num, const double* den, int op, TileMap tm) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= N) return;
+  if (n >= g.N) return;
   double sc = 1.0;
   if (num && den) sc = (op == 0) ? (*num / *den) : (*den / *num);
   if (op == 0)
-    d[n] = (TD)((double)src[n] * sc);
+    d[tmap(tm, g, n)] = (TD)((double)src[n] * sc);
   else
     d[n] = (TD)((double)d[n] + (double)src[n] * sc);
 }
 
 template <typename TD, typename TS>
-void launch_cast_scale(uint32_t N, TD* dst, const TS* src, const double* num, const double* den,
-                       int op, hipStream_t s) {
-  k_cast_scale<TD, TS><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(N, dst, src, num, den, op);
+void launch_cast_scale(const Grid3& g, TD* dst, const TS* src, const double* num,
+                       const double* den, int op, hipStream_t s, const TileMap& tm) {
+  k_cast_scale<TD, TS><<<(g.N + kBlock - 1) / kBlock, kBlock, 0, s>>>(g, dst, src, num, den, op,
+                                                                      tm);
 }
 
 __global__ void k_fill(double* p, double v, size_t n) {
@@ -993,26 +1010,31 @@ void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi
                                   double*, bool, hipStream_t);                                  \
   template void launch_cg_update<T>(const Metric&, const Grid3&, const Launch&, int, T*, T*,    \
                                     const T*, const T*, CgState*, Reduce, int, hipStream_t,     \
-                                    double*, const double*);                                    \
+                                    double*, const double*, const TileMap&);                    \
   template void launch_p_update<T>(const Metric&, const Grid3&, const Launch&, int, T*,         \
-                                   const T*, const T*, CgState*, hipStream_t);                  \
+                                   const T*, const T*, CgState*, hipStream_t, const TileMap&);  \
   template void launch_dot2<T>(const Grid3&, const Launch&, const T*, const T*, const T*,       \
                                const T*, CgState*, Reduce, int, Epi, double*, bool,             \
-                               hipStream_t);                                                    \
+                               hipStream_t, const TileMap&);                                    \
   template void launch_point_prec<T>(const Metric&, const Grid3&, const Launch&, int, const T*, \
                                      T*, CgState*, Reduce, int, Epi, double*, hipStream_t);     \
-  template void launch_sub<T>(const Grid3&, const Launch&, T*, const T*, hipStream_t);          \
-  template void launch_copy<T>(const Grid3&, const Launch&, T*, const T*, hipStream_t);         \
+  template void launch_sub<T>(const Grid3&, const Launch&, T*, const T*, hipStream_t,           \
+                              const TileMap&);                                                  \
+  template void launch_copy<T>(const Grid3&, const Launch&, T*, const T*, hipStream_t,          \
+                               const TileMap&, int);                                            \
   template void launch_dilu_solve_levels<T>(const Metric&, const Grid3&, const float*,          \
                                             const T*, float*, T*, const CoefArrays*, bool,      \
                                             const IluBlocks&, const CgState*, hipStream_t);
 P3D_INST(double)
 P3D_INST(float)
-template void launch_cast_scale<float, double>(uint32_t, float*, const double*, const double*,
-                                               const double*, int, hipStream_t);
-template void launch_cast_scale<double, float>(uint32_t, double*, const float*, const double*,
-                                               const double*, int, hipStream_t);
-template void launch_cast_scale<double, double>(uint32_t, double*, const double*, const double*,
-                                                const double*, int, hipStream_t);
+template void launch_cast_scale<float, double>(const Grid3&, float*, const double*,
+                                               const double*, const double*, int, hipStream_t,
+                                               const TileMap&);
+template void launch_cast_scale<double, float>(const Grid3&, double*, const float*,
+                                               const double*, const double*, int, hipStream_t,
+                                               const TileMap&);
+template void launch_cast_scale<double, double>(const Grid3&, double*, const double*,
+                                                const double*, const double*, int, hipStream_t,
+                                                const TileMap&);
 
 }  // namespace p3d

@@ -119,18 +119,22 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
 template <typename T>
 void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
-                      hipStream_t s, double* xs = nullptr, const double* xscale = nullptr);
+                      hipStream_t s, double* xs = nullptr, const double* xscale = nullptr,
+                      const TileMap& tm = TileMap{});
 
 // p = z + beta p (z = M^-1 r computed in place for DIAG/NONE, read for ILU).
+// tm: placement of r (cg_update) and z (p_update) (natural by default).
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
-                     const T* r, const T* z, CgState* st, hipStream_t s);
+                     const T* r, const T* z, CgState* st, hipStream_t s,
+                     const TileMap& tm = TileMap{});
 
 // Generic fused dot: s0 = x.y, s1 = u.v (v/u may be null), epilogue epi.
+// All four vectors are placed by tm; the sums run in natural point order.
 template <typename T>
 void launch_dot2(const Grid3& g, const Launch& L, const T* x, const T* y, const T* u,
                  const T* v, CgState* st, Reduce red, int slot, Epi epi, double* out,
-                 bool guard, hipStream_t s);
+                 bool guard, hipStream_t s, const TileMap& tm = TileMap{});
 
 // z = M^-1 r for DIAG/NONE (r,z may alias), with optional s0=r.z, s1=r.r.
 template <typename T>
@@ -138,16 +142,22 @@ void launch_point_prec(const Metric& m, const Grid3& g, const Launch& L, int pre
                        T* z, CgState* st, Reduce red, int slot, Epi epi, double* out,
                        hipStream_t s);
 
-// Residual init: r = r - ap, and p = z later.
+// Residual init: r = r - ap (r placed by tm), and p = z later.
 template <typename T>
-void launch_sub(const Grid3& g, const Launch& L, T* r, const T* ap, hipStream_t s);
+void launch_sub(const Grid3& g, const Launch& L, T* r, const T* ap, hipStream_t s,
+                const TileMap& tm = TileMap{});
 
+// dst = src over the N points; with a tile map, gather (dst natural, src
+// placed by tm) or scatter (scatter = 1: dst placed by tm, src natural).
 template <typename T>
-void launch_copy(const Grid3& g, const Launch& L, T* dst, const T* src, hipStream_t s);
+void launch_copy(const Grid3& g, const Launch& L, T* dst, const T* src, hipStream_t s,
+                 const TileMap& tm = TileMap{}, int scatter = 0);
 
+// op 0: dst = src * num/den, dst placed by tm; op 1: dst += src * den/num.
 template <typename TD, typename TS>
-void launch_cast_scale(uint32_t N, TD* dst, const TS* src, const double* scale_num,
-                       const double* scale_den, int op, hipStream_t s);
+void launch_cast_scale(const Grid3& g, TD* dst, const TS* src, const double* scale_num,
+                       const double* scale_den, int op, hipStream_t s,
+                       const TileMap& tm = TileMap{});
 
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
 

@@ -77,6 +77,10 @@ struct p3d_solver {
   float* yv = nullptr;
   float* dinv = nullptr;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr;
+  // r, z (and r32, z32) hold nvec >= N entries: with the single-wave tile
+  // sweeps they are tile-major (tm.on), every other vector is natural
+  size_t nvec = 0;
+  TileMap tm;
   double* pole_part = nullptr;
   CgState* st = nullptr;
   CgState* st_host = nullptr;  // pinned
@@ -122,12 +126,42 @@ static int dalloc(void** p, size_t bytes) {
 
 static int ensure_mixed(p3d_solver* s) {
   if (s->x32) return P3D_OK;
-  const size_t nb = (size_t)s->g.N * sizeof(float);
+  const size_t nb = (size_t)s->g.N * sizeof(float), nv = s->nvec * sizeof(float);
   int rc;
-  if ((rc = dalloc((void**)&s->x32, nb)) || (rc = dalloc((void**)&s->r32, nb)) ||
+  if ((rc = dalloc((void**)&s->x32, nb)) || (rc = dalloc((void**)&s->r32, nv)) ||
       (rc = dalloc((void**)&s->p32, nb)) || (rc = dalloc((void**)&s->ap32, nb)) ||
-      (rc = dalloc((void**)&s->z32, nb)))
+      (rc = dalloc((void**)&s->z32, nv)))
     return rc;
+  HIPCHK(hipMemsetAsync(s->r32, 0, nv, s->stream));
+  HIPCHK(hipMemsetAsync(s->z32, 0, nv, s->stream));
+  return P3D_OK;
+}
+
+// Placement of r and z: tile-major exactly when the single-wave tile
+// sweeps apply the ILU.  A change of placement drops the captured graphs
+// and clears r and z (their entries outside the grid stay zero).
+static int update_tilemap(p3d_solver* s) {
+  TileMap t;
+  if (s->ifprec == PREC_ILU && s->sweep_mode == 0 && s->sweep_ready && s->sweep.SI == 1 &&
+      s->sweep.SJ == 1) {
+    t.on = 1;
+    t.TNI = s->sweep.TNI;
+    t.TNJ = s->sweep.TNJ;
+    t.tk_log2 = s->sweep.TK == 16 ? 4 : (s->sweep.TK == 64 ? 6 : 5);
+  }
+  if (t.on == s->tm.on && t.TNI == s->tm.TNI && t.TNJ == s->tm.TNJ && t.tk_log2 == s->tm.tk_log2)
+    return P3D_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int q = 0; q < 3; q++) {
+    if (s->gexec[q]) hipGraphExecDestroy(s->gexec[q]);
+    s->gexec[q] = nullptr;
+  }
+  s->tm = t;
+  HIPCHK(hipMemsetAsync(s->r, 0, s->nvec * sizeof(double), s->stream));
+  HIPCHK(hipMemsetAsync(s->z, 0, s->nvec * sizeof(double), s->stream));
+  if (s->r32) HIPCHK(hipMemsetAsync(s->r32, 0, s->nvec * sizeof(float), s->stream));
+  if (s->z32) HIPCHK(hipMemsetAsync(s->z32, 0, s->nvec * sizeof(float), s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return P3D_OK;
 }
 
@@ -170,7 +204,7 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
     if (epi != EPI_NONE)
       launch_dot2<T>(s->g, s->L, r, z, (epi == EPI_RZITER) ? nullptr : r,
                      (epi == EPI_RZITER) ? nullptr : r, s->st, s->red, 1, epi, out, guard,
-                     s->stream);
+                     s->stream, s->tm);
   } else {
     launch_point_prec<T>(s->m, s->g, s->L, s->ifprec, r, z, s->st, s->red, 1, epi, out,
                          s->stream);
@@ -183,9 +217,9 @@ static void cg_iteration(p3d_solver* s, Vecs<T> v) {
   launch_stencil<T>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red, 0,
                     EPI_PAP, nullptr, true, s->stream);
   launch_cg_update<T>(s->m, s->g, s->L, s->ifprec, v.x, v.r, v.p, v.ap, s->st, s->red, 2,
-                      s->stream, v.xs, v.xscale);
+                      s->stream, v.xs, v.xscale, s->tm);
   if (s->ifprec == PREC_ILU) prec_and_dot<T>(s, v.r, v.z, EPI_RZITER, nullptr, true);
-  launch_p_update<T>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream);
+  launch_p_update<T>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm);
 }
 
 template <typename T>
@@ -276,9 +310,9 @@ static int pcg(p3d_solver* s, Vecs<T> v, double epscg, int ncgmax, int stop_mode
   launch_pole_sums<T>(s->m, s->g, v.x, s->pole_part, s->stream);
   launch_stencil<T>(s->m, s->g, s->L, v.x, v.ap, nullptr, 0, s->pole_part, nullptr, s->red, 0,
                     EPI_NONE, nullptr, false, s->stream);
-  launch_sub<T>(s->g, s->L, v.r, v.ap, s->stream);
+  launch_sub<T>(s->g, s->L, v.r, v.ap, s->stream, s->tm);
   prec_and_dot<T>(s, v.r, v.z, EPI_RINIT, nullptr, false);
-  launch_copy<T>(s->g, s->L, v.p, v.z, s->stream);
+  launch_copy<T>(s->g, s->L, v.p, v.z, s->stream, s->tm, 0);
   if ((rc = read_state(s))) return rc;
   return run_until_done<T>(s, v);
 }
@@ -350,7 +384,8 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
     s->out_host[5] = 1.0;
     HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, 2 * sizeof(double), hipMemcpyHostToDevice,
                           s->stream));
-    launch_cast_scale<float, double>(s->g.N, s->r32, s->ap, s->out + 5, s->out + 4, 0, s->stream);
+    launch_cast_scale<float, double>(s->g, s->r32, s->ap, s->out + 5, s->out + 4, 0, s->stream,
+                                     s->tm);
     HIPCHK(hipMemsetAsync(s->x32, 0, sizeof(float) * s->g.N, s->stream));
     double tol = s->inner_tol;
     const double need = 0.5 * epscg / relres;
@@ -362,7 +397,7 @@ static int solve_mixed(p3d_solver* s, double epscg, int ncgmax) {
       return fail(P3D_ENONFINITE, "non-finite residual in the inner solve");
     }
     // x += x32 * scale
-    launch_cast_scale<double, float>(s->g.N, s->x, s->x32, s->out + 5, s->out + 4, 1, s->stream);
+    launch_cast_scale<double, float>(s->g, s->x, s->x32, s->out + 5, s->out + 4, 1, s->stream);
     if ((rc = true_residual(s, &rr, &bb))) return rc;
     relres = sqrt(rr / bb);
     outer++;
@@ -418,7 +453,8 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
                         s->stream));
   // the recurrence's solution updates go straight to the FP64 x
   Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32, s->x, s->out + 4};
-  launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
+  launch_cast_scale<float, double>(s->g, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream,
+                                   s->tm);
   // the inner relres = ||r32|| scale / ||b|| estimates the true one
   const double final_tgt = 0.8 * epscg;
   CgState init;
@@ -433,7 +469,7 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
   *s->st_host = init;
   HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
   prec_and_dot<float>(s, v.r, v.z, EPI_RINIT, nullptr, false);
-  launch_copy<float>(s->g, s->L, v.p, v.z, s->stream);
+  launch_copy<float>(s->g, s->L, v.p, v.z, s->stream, s->tm, 0);
   if ((rc = read_state(s))) return rc;
   int replacements = 0;
   for (;;) {
@@ -465,13 +501,14 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
     }
     // replace the residual and continue the recurrence: r32 = ap/scale ;
     // z = M^-1 r32 ; beta = r.z / (r.z of the last direction) ; p = z + beta p
-    launch_cast_scale<float, double>(s->g.N, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream);
+    launch_cast_scale<float, double>(s->g, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream,
+                                   s->tm);
     s->st_host->epscg = fmax(kReliableDelta * relres, final_tgt);
     s->st_host->done = 0;
     HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice,
                           s->stream));
     prec_and_dot<float>(s, v.r, v.z, EPI_REPLACE, nullptr, false);
-    launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream);
+    launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm);
     if ((rc = read_state(s))) return rc;
   }
   s->stats.relres = s->stats.true_relres = relres;
@@ -520,8 +557,7 @@ static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
     rc = s->mixed == 2 ? solve_mixed_ru(s, epscg, ncgmax) : solve_mixed(s, epscg, ncgmax);
     HIPCHK(hipStreamSynchronize(s->stream));
   } else {
-    HIPCHK(hipMemcpyAsync(s->r, s->b, sizeof(double) * s->g.N, hipMemcpyDeviceToDevice,
-                          s->stream));
+    launch_copy<double>(s->g, s->L, s->r, s->b, s->stream, s->tm, 1);
     Vecs<double> v{s->x, s->r, s->p, s->ap, s->z};
     if ((rc = pcg<double>(s, v, epscg, ncgmax, s->stop_mode))) return rc;
     print_history(s, ncghist);
@@ -693,10 +729,19 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   s->em.dph = s->mesh + off[7];
   // vectors
   const size_t nb = (size_t)N * sizeof(double);
-  if ((rc = dalloc((void**)&s->x, nb)) || (rc = dalloc((void**)&s->r, nb)) ||
+  // r and z also hold the tile-major placement: 8x8 columns x 64 planes
+  {
+    const size_t tiled = (size_t)((a + 7) / 8 * 8) * (size_t)((b + 7) / 8 * 8) *
+                         (size_t)((c + 63) / 64 * 64);
+    s->nvec = tiled > (size_t)N ? tiled : (size_t)N;
+  }
+  const size_t nbv = s->nvec * sizeof(double);
+  if ((rc = dalloc((void**)&s->x, nb)) || (rc = dalloc((void**)&s->r, nbv)) ||
       (rc = dalloc((void**)&s->p, nb)) || (rc = dalloc((void**)&s->ap, nb)) ||
-      (rc = dalloc((void**)&s->z, nb)) || (rc = dalloc((void**)&s->b, nb)))
+      (rc = dalloc((void**)&s->z, nbv)) || (rc = dalloc((void**)&s->b, nb)))
     return bail(rc);
+  hipMemset(s->r, 0, nbv);
+  hipMemset(s->z, 0, nbv);
   if ((rc = dalloc((void**)&s->br0, sizeof(double) * nt * np))) return bail(rc);
   if ((rc = dalloc((void**)&s->pole_part, sizeof(double) * 2 * kPoleSegs * a))) return bail(rc);
   if ((rc = dalloc((void**)&s->st, sizeof(CgState)))) return bail(rc);
@@ -726,6 +771,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
     s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
                                        &s->blocks, s->dinv) == 0);
+    if ((rc = update_tilemap(s))) return bail(rc);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -776,9 +822,10 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
   const int prc =
       build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv);
   s->sweep_ready = prc == 0;
+  int rc = update_tilemap(s);
   if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
   if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");
-  return P3D_OK;
+  return rc;
 }
 
 int p3d_set_option(p3d_solver* s, int key, double value) {
@@ -811,7 +858,7 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
         if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
         s->gexec[t] = nullptr;
       }
-      break;
+      return update_tilemap(s);
     case P3D_OPT_VERBOSE:
       s->verbose = value != 0;
       break;
@@ -900,9 +947,12 @@ int p3d_solve_dev(p3d_solver* s, double* x_dev, double* rhs_dev, double epscg, i
   HIPCHK(hipMemcpyAsync(s->x, x_dev, nb, hipMemcpyDeviceToDevice, s->stream));
   rc = solve_core(s, epscg, ncgmax, 0);
   HIPCHK(hipMemcpyAsync(x_dev, s->x, nb, hipMemcpyDeviceToDevice, s->stream));
-  if (rhs_dev)
-    HIPCHK(hipMemcpyAsync(rhs_dev, s->mixed ? s->ap : s->r, nb, hipMemcpyDeviceToDevice,
-                          s->stream));
+  if (rhs_dev) {
+    // the final residual: FP64 recurrence r (natural order via ap), or the
+    // true residual the mixed solve left in ap
+    if (!s->mixed) launch_copy<double>(s->g, s->L, s->ap, s->r, s->stream, s->tm, 0);
+    HIPCHK(hipMemcpyAsync(rhs_dev, s->ap, nb, hipMemcpyDeviceToDevice, s->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   if (ncg) *ncg = s->stats.ncg;
   if (epsn) *epsn = s->stats.epsn;
@@ -923,8 +973,10 @@ int p3d_solve(p3d_solver* s, double* x, double* rhs, double epscg, int ncgmax, i
   HIPCHK(hipMemcpyAsync(s->x, x, nb, hipMemcpyHostToDevice, s->stream));
   rc = solve_core(s, epscg, ncgmax, ncghist);
   HIPCHK(hipMemcpyAsync(x, s->x, nb, hipMemcpyDeviceToHost, s->stream));
-  if (rhs)
-    HIPCHK(hipMemcpyAsync(rhs, s->mixed ? s->ap : s->r, nb, hipMemcpyDeviceToHost, s->stream));
+  if (rhs) {
+    if (!s->mixed) launch_copy<double>(s->g, s->L, s->ap, s->r, s->stream, s->tm, 0);
+    HIPCHK(hipMemcpyAsync(rhs, s->ap, nb, hipMemcpyDeviceToHost, s->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   if (ncg) *ncg = s->stats.ncg;
   if (epsn) *epsn = s->stats.epsn;
@@ -1029,10 +1081,13 @@ int p3d_apply_A(p3d_solver* s, const double* x, double* y) {
 int p3d_apply_prec(p3d_solver* s, double* x) {
   if (!s || !x) return fail(P3D_EINVAL, "null argument");
   const size_t nb = sizeof(double) * s->g.N;
-  HIPCHK(hipMemcpyAsync(s->r, x, nb, hipMemcpyHostToDevice, s->stream));
+  // x -> r and z -> x through the natural-order staging vector ap
+  HIPCHK(hipMemcpyAsync(s->ap, x, nb, hipMemcpyHostToDevice, s->stream));
+  launch_copy<double>(s->g, s->L, s->r, s->ap, s->stream, s->tm, 1);
   prec_and_dot<double>(s, s->r, s->z, EPI_NONE, nullptr, false);
+  launch_copy<double>(s->g, s->L, s->ap, s->z, s->stream, s->tm, 0);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(x, s->z, nb, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(x, s->ap, nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (s->ifprec == PREC_ILU && sweep_error(s->sweep))
     return fail(P3D_EHIP, "ILU dataflow sweep exceeded its wait bound (results invalid)");
@@ -1108,11 +1163,11 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
         break;
       case 4:
         launch_cg_update<double>(s->m, s->g, s->L, s->ifprec, s->x, s->r, s->p, s->ap, s->st,
-                                 s->red, 2, s->stream);
+                                 s->red, 2, s->stream, nullptr, nullptr, s->tm);
         break;
       case 5:
         launch_p_update<double>(s->m, s->g, s->L, s->ifprec, s->p, s->r, s->z, s->st,
-                                s->stream);
+                                s->stream, s->tm);
         break;
       case 6:  // plain copy of N doubles: the bandwidth reference
         launch_copy<double>(s->g, s->L, s->ap, s->p, s->stream);

@@ -597,6 +597,8 @@ struct TileLds {
   float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
 
+// r, z, dinv and the FP32 sweep vector are all tile-major (TileMap): a
+// tile reads and writes whole 64-point planes.
 // TK/8 waves stage the tile (8 planes each, one round of loads in flight
 // per wave) and its float couplings; wave 0 marches; all waves store.  The
 // faces a tile needs from its lower (forward) or upper (backward)
@@ -621,7 +623,6 @@ __global__ void __launch_bounds__(8 * TK)
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
-  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
   // couplings kept by the block-Jacobi cut (tile-aligned: inside a tile only
   // the first plane can start a phi block)
   const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
@@ -660,7 +661,7 @@ __global__ void __launch_bounds__(8 * TK)
       rv[u] = dv[u] = 0.f;
       if (q < nk) {
         dv[u] = da.dinv_t[tb + (size_t)q * 64];
-        if (act) rv[u] = (float)r[col + g.ab * (uint32_t)(k0 + q)];
+        rv[u] = (float)r[tb + (size_t)q * 64];
       }
     }
     if (done) return;  // uniform over the block
@@ -767,7 +768,6 @@ __global__ void __launch_bounds__(8 * TK)
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
-  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
   const bool hi_i = da.bl.hi_i(i, g.a), hi_j = da.bl.hi_j(j, g.b);
   const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
@@ -881,7 +881,7 @@ __global__ void __launch_bounds__(8 * TK)
       if (q < nk) {
         const float zz = sh.v[q][lane];
         da.yv_t[tb + (size_t)q * 64] = zz;
-        z[col + g.ab * (uint32_t)(k0 + q)] = (T)zz;
+        z[tb + (size_t)q * 64] = (T)zz;
       }
     }
   }
