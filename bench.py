@@ -183,6 +183,10 @@ def main():
     ap.add_argument("--blocks", type=str, default="2,8,8",
                     help="block-Jacobi ILU0 blocks (r,theta,phi) of the fast paths; "
                          "1,1,1 = single-rank ILU0")
+    ap.add_argument("--tile-k", type=int, default=16,
+                    help="planes per ILU tile of the block-Jacobi fast paths (16, 32, 64)")
+    ap.add_argument("--no-levels", action="store_true",
+                    help="skip the FP64 run with hyperplane-level trisolves")
     ap.add_argument("--guess-check", type=int, default=2,
                     help="1 residual rule, 2 A-norm rule for accepting the learned guess")
     args = ap.parse_args()
@@ -264,6 +268,17 @@ def main():
     # FP64 plain ILU0 PCG (drop-in baseline: the single-rank ILU0)
     t64, st64, x64 = timed("fp64 ILU0 PCG", 0)
     w64 = s.magnetic_energy(s.fill_phi(x64))
+    # the same solve with the triangular solves level-scheduled by i+j+k
+    # hyperplanes (one launch per hyperplane): the structure of the
+    # reference's GPU ILU0 path; one untimed-warm-up-free run
+    tlv = None
+    if not args.no_levels:
+        s.set_option(_lib.OPT_SWEEP, 1)
+        dt, stlv = run(0)
+        tlv = (dt, stlv["ncg"])
+        print(f"[bench] fp64 ILU0 PCG, hyperplane-level trisolves: {dt:.3f}s ncg={stlv['ncg']}",
+              file=sys.stderr, flush=True)
+        s.set_option(_lib.OPT_SWEEP, 0)
     kern = measure_kernels(s, N, nr * nt * np_)
     # the fast paths use the block-Jacobi ILU0 (the reference's per-rank ILU0
     # on a grid decomposed into that many blocks)
@@ -271,6 +286,7 @@ def main():
     for key, v in zip((_lib.OPT_ILU_BLOCKS_R, _lib.OPT_ILU_BLOCKS_T, _lib.OPT_ILU_BLOCKS_P),
                       blocks):
         s.set_option(key, v)
+    s.set_option(_lib.OPT_TILE_K, args.tile_k)
     ilu_bj = {"fp64_ms": round(s.time_kernel(2, 5), 5), "fp32_ms": round(s.time_kernel(7, 5), 5)}
     tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0)
     # mixed precision from the zero guess
@@ -342,11 +358,11 @@ def main():
         sc.close()
         hip.free(d_phic)
 
-    dom = kern["dominant"]
-    roof = {"kernel": dom["name"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(w.name, dom["name"]),
-            "avg_ms": dom["ms"], "bytes_per_launch": dom["bytes"]}
+    dk = kern["dominant"]
+    roof = {"kernel": dk["name"], "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(w.name, dk["name"]),
+            "avg_ms": dk["ms"], "bytes_per_launch": dk["bytes"]}
     st_name = "stencil_fp64 (k_stencil_march<double>)"
     per_iter64 = min(t64) / max(st64["ncg"], 1)
     cpu = None
@@ -370,10 +386,13 @@ def main():
                    "epscg": 1e-9,
                    "value_path": label, "mixed_mode": args.mixed_mode},
         "fp64_plain_s": round(min(t64), 4), "fp64_plain_ncg": st64["ncg"],
-        "ilu_blocks": list(blocks), "ilu_apply_blockjacobi": ilu_bj,
+        "ilu_blocks": list(blocks), "ilu_tile_k": args.tile_k, "ilu_apply_blockjacobi": ilu_bj,
         "fp64_blockjacobi_s": round(min(tbj), 4), "fp64_blockjacobi_ncg": stbj["ncg"],
         "mixed_s": round(min(tmx), 4), "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
         "speedup_vs_fp64": round(min(t64) / best, 3),
+        "fp64_levels_s": None if tlv is None else round(tlv[0], 4),
+        "fp64_levels_ncg": None if tlv is None else tlv[1],
+        "speedup_vs_fp64_levels": None if tlv is None else round(tlv[0] / best, 3),
         "mixed_errors_vs_fp64": errs(xmx),
         "learned": result_ws,
         "fp64_ms_per_iter": round(1e3 * per_iter64, 4),
