@@ -185,6 +185,9 @@ def main():
                          "1,1,1 = single-rank ILU0")
     ap.add_argument("--tile-k", type=int, default=16,
                     help="planes per ILU tile of the block-Jacobi fast paths (16, 32, 64)")
+    ap.add_argument("--tile-stage", type=int, default=1,
+                    help="ILU tiles of the fast paths staged by the marching wave alone (1) "
+                         "or by planes/8 waves (0)")
     ap.add_argument("--no-levels", action="store_true",
                     help="skip the FP64 run with hyperplane-level trisolves")
     ap.add_argument("--guess-check", type=int, default=2,
@@ -287,6 +290,7 @@ def main():
                       blocks):
         s.set_option(key, v)
     s.set_option(_lib.OPT_TILE_K, args.tile_k)
+    s.set_option(_lib.OPT_TILE_STAGE, args.tile_stage)
     ilu_bj = {"fp64_ms": round(s.time_kernel(2, 5), 5), "fp32_ms": round(s.time_kernel(7, 5), 5)}
     tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0)
     # mixed precision from the zero guess
@@ -386,7 +390,7 @@ def main():
                    "epscg": 1e-9,
                    "value_path": label, "mixed_mode": args.mixed_mode},
         "fp64_plain_s": round(min(t64), 4), "fp64_plain_ncg": st64["ncg"],
-        "ilu_blocks": list(blocks), "ilu_tile_k": args.tile_k, "ilu_apply_blockjacobi": ilu_bj,
+        "ilu_blocks": list(blocks), "ilu_tile_k": args.tile_k, "ilu_tile_stage": args.tile_stage, "ilu_apply_blockjacobi": ilu_bj,
         "fp64_blockjacobi_s": round(min(tbj), 4), "fp64_blockjacobi_ncg": stbj["ncg"],
         "mixed_s": round(min(tmx), 4), "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
         "speedup_vs_fp64": round(min(t64) / best, 3),

@@ -116,11 +116,16 @@ enum {
                              most 32 planes) */
   P3D_OPT_ILU_BLOCKS_R = 11, /* block-Jacobi ILU0: blocks along r, theta, */
   P3D_OPT_ILU_BLOCKS_T = 12, /* phi (default 1 each: the single-rank ILU0). */
-  P3D_OPT_ILU_BLOCKS_P = 13  /* Couplings between blocks are dropped, as in
+  P3D_OPT_ILU_BLOCKS_P = 13, /* Couplings between blocks are dropped, as in
                                 the reference's ILU0 on a grid decomposed
                                 over that many ranks (src/pot3d.F90:4940-
                                 4967); block extents are rounded up to 16,
                                 32 and 64 points. */
+  P3D_OPT_TILE_STAGE = 14   /* 8x8-column tiles: 0 planes/8 waves load each
+                                tile and one marches (default); 1 the
+                                marching wave loads its tile alone (one wave
+                                per workgroup, more tiles resident per CU).
+                                Same results bit for bit. */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

@@ -104,7 +104,7 @@ struct p3d_solver {
   int sweep_mode = 0;
   int verbose = 0;
   int guess_check = 0;
-  int tile_k = 32, tile_waves = 1;
+  int tile_k = 32, tile_waves = 1, tile_stage = 0;
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
   // graphs: [0] FP64, [1] FP32, [2] FP32 recurrence with the FP64 solution
@@ -770,7 +770,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     hipFree(dpiv);
     if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
     s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
-                                       &s->blocks, s->dinv) == 0);
+                                       &s->blocks, s->dinv, s->tile_stage) == 0);
     if ((rc = update_tilemap(s))) return bail(rc);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -820,7 +820,8 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int prc =
-      build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv);
+      build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv,
+                       s->tile_stage);
   s->sweep_ready = prc == 0;
   int rc = update_tilemap(s);
   if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
@@ -863,14 +864,18 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       s->verbose = value != 0;
       break;
     case P3D_OPT_TILE_K:
-    case P3D_OPT_TILE_WAVES: {
+    case P3D_OPT_TILE_WAVES:
+    case P3D_OPT_TILE_STAGE: {
       const int v = (int)value;
       if (key == P3D_OPT_TILE_K && v != 16 && v != 32 && v != 64)
         return fail(P3D_EINVAL, "P3D_OPT_TILE_K must be 16, 32 or 64");
       if (key == P3D_OPT_TILE_WAVES && v != 1 && v != 4 && v != 8)
         return fail(P3D_EINVAL, "P3D_OPT_TILE_WAVES must be 1, 4 or 8");
+      if (key == P3D_OPT_TILE_STAGE && v != 0 && v != 1)
+        return fail(P3D_EINVAL, "P3D_OPT_TILE_STAGE must be 0 or 1");
       if (key == P3D_OPT_TILE_K) s->tile_k = v;
-      else s->tile_waves = v;
+      else if (key == P3D_OPT_TILE_WAVES) s->tile_waves = v;
+      else s->tile_stage = v;
       if (s->ifprec != PREC_ILU) break;
       return rebuild_ilu(s, false);
     }

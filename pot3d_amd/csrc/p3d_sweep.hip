@@ -546,7 +546,9 @@ struct DiagArgs {
   float* face_i;     // per tile [TK][8]: its last (forward) / first (backward)
   float* face_j;     //   column (face_i) and row (face_j) values, for the next tile
   const float* dinv_t;  // dinv, tile-major (tile, plane, lane); zero outside the grid
-  float* yv_t;          // forward result y / backward z in FP32, tile-major
+  float* yv_t;          // forward result y in FP32, tile-major
+  float* face_k;        // per tile [64]: backward z of its first plane (for the tile below)
+  float* face_kt;       // per column of tiles [64]: backward z of plane c-1 (phi seam)
 };
 
 __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
@@ -577,44 +579,58 @@ __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int
 static DiagArgs diag_args(const SweepPlan& plan, int L) {
   return DiagArgs{L,       plan.TNI, plan.TNJ,    plan.NK,
                   plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr,
-                  plan.bl, plan.face_i, plan.face_j, plan.dinv_t, plan.yv_t};
+                  plan.bl, plan.face_i, plan.face_j, plan.dinv_t, plan.yv_t,
+                  plan.face_k, plan.face_kt};
 }
 
 __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int Kc) {
   return (size_t)I + (size_t)da.NI * ((size_t)J + (size_t)da.NJ * (size_t)Kc);
 }
 
-template <int TK>
+// LDS of one 8x8xTK tile.  With couplings staged (STAGED), the staging waves
+// form the float couplings of every plane and lane; otherwise the marching
+// wave forms them from the per-plane factors (less LDS per tile, so more
+// tiles resident per CU).
+template <int TK, bool STAGED>
 struct TileLds {
   // per plane and lane; row TK is scratch, written by lanes whose step
   // falls outside the tile's planes (keeps the march free of branches)
-  float v[TK + 1][64];   // r (forward) or y (backward), overwritten by the result
-  float d[TK + 1][64];   // dinv
-  float ck[TK + 1][64];  // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
-  float cj[TK + 1][64];
-  float ci[TK + 1][64];
+  float v[TK + 1][64];  // r (forward) or y (backward), overwritten by the result
+  float d[TK + 1][64];  // dinv
+  // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
+  float ck[STAGED ? TK + 1 : 1][64];
+  float cj[STAGED ? TK + 1 : 1][64];
+  float ci[STAGED ? TK + 1 : 1][64];
+  double dph[STAGED ? 1 : TK + 1], dpx[STAGED ? 1 : TK + 1];  // DPH and DPL / DPU
   float fi[TK + 1][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
   float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
 
 // r, z, dinv and the FP32 sweep vector are all tile-major (TileMap): a
 // tile reads and writes whole 64-point planes.
-// TK/8 waves stage the tile (8 planes each, one round of loads in flight
-// per wave) and its float couplings; wave 0 marches; all waves store.  The
-// faces a tile needs from its lower (forward) or upper (backward)
-// neighbours in i and j are not read from the scattered neighbour points:
-// every tile leaves its boundary column and row in compact per-tile
-// buffers (face_i, face_j; 8 values per plane), which the next tile reads
-// as one contiguous block per face.  The march step is branch-free: lane
-// data moves by DPP / permlane (no LDS round trip on the dependency
-// chain), and its operands for 8 steps are read from LDS in one batch.
-template <typename T, int TK>
-__global__ void __launch_bounds__(8 * TK)
+// W waves per tile: with W = TK/8 every wave stages 8 planes (one round of
+// loads in flight per wave) and their float couplings, wave 0 marches and
+// all waves store; with W = 1 the marching wave stages the whole tile
+// itself, in rounds of up to 16 planes, and forms the couplings during the
+// march (a workgroup is one wave and about 20 KB of LDS, so several tiles
+// march on one CU at once).  The faces a tile needs from its lower
+// (forward) or upper (backward) neighbours in i and j are not read from
+// the scattered neighbour points: every tile leaves its boundary column and
+// row in compact per-tile buffers (face_i, face_j; 8 values per plane),
+// which the next tile reads as one contiguous block per face.  The march
+// step is branch-free: lane data moves by DPP / permlane (no LDS round trip
+// on the dependency chain), and its operands for 8 steps are read from LDS
+// in one batch.
+template <typename T, int TK, int W>
+__global__ void __launch_bounds__(64 * W)
     k_tile_fwd_lds(Metric m, Grid3 g, DiagArgs da, const T* __restrict__ r,
                    const CgState* guard) {
+  constexpr bool kStaged = W > 1;
+  constexpr int kQPW = TK / W;                    // planes staged per wave
+  constexpr int kRound = kQPW < 16 ? kQPW : 16;   // planes per round of loads
   // the done flag is read with the tile's data and tested before any store
   const int done = guard ? st_done(guard) : 0;
-  __shared__ TileLds<TK> sh;
+  __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
   const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
@@ -644,20 +660,13 @@ __global__ void __launch_bounds__(8 * TK)
       t_w = da.dinv_t[q] * da.yv_t[q];
     }
   }
-  // incoming faces: thread -> (plane fq, index fe), one float each
-  const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
-  float fi_s = 0.f, fj_s = 0.f;
-  if (fq < nk) {
-    if (I > 0 && da.bl.lo_i(I * TI))
-      fi_s = da.face_i[(tile_id(da, I - 1, J, Kc) * TK + fq) * 8 + fe];
-    if (J > 0 && da.bl.lo_j(J * TJ))
-      fj_s = da.face_j[(tile_id(da, I, J - 1, Kc) * TK + fq) * 8 + fe];
-  }
-  {
-    float rv[8], dv[8];
+  // stage: planes wave*kQPW .. +kQPW-1, rounds of kRound
+#pragma unroll 1
+  for (int q0 = wave * kQPW; q0 < (wave + 1) * kQPW; q0 += kRound) {
+    float rv[kRound], dv[kRound];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
+    for (int u = 0; u < kRound; u++) {
+      const int q = q0 + u;
       rv[u] = dv[u] = 0.f;
       if (q < nk) {
         dv[u] = da.dinv_t[tb + (size_t)q * 64];
@@ -666,21 +675,32 @@ __global__ void __launch_bounds__(8 * TK)
     }
     if (done) return;  // uniform over the block
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
+    for (int u = 0; u < kRound; u++) {
+      const int q = q0 + u;
       if (q < nk) {
-        const double dph = m.DPH[k0 + q];
         sh.v[q][lane] = rv[u];
         sh.d[q][lane] = dv[u];
-        sh.ck[q][lane] = (float)(-(P1 * m.DPL[k0 + q]));
-        sh.cj[q][lane] = (float)(-((DRH * dph) * CTL));
-        sh.ci[q][lane] = (float)(-((SD * dph) * CRL));
+        if constexpr (kStaged) {
+          const double dph = m.DPH[k0 + q];
+          sh.ck[q][lane] = (float)(-(P1 * m.DPL[k0 + q]));
+          sh.cj[q][lane] = (float)(-((DRH * dph) * CTL));
+          sh.ci[q][lane] = (float)(-((SD * dph) * CRL));
+        }
       }
     }
   }
-  if (fq < nk) {
-    sh.fi[fq][fe] = fi_s;
-    sh.fj[fq][fe] = fj_s;
+  // incoming faces (one float per thread and round) and per-plane factors
+  const bool face_i = I > 0 && da.bl.lo_i(I * TI), face_j = J > 0 && da.bl.lo_j(J * TJ);
+  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
+    const int fq = f >> 3, fe = f & 7;
+    sh.fi[fq][fe] = face_i ? da.face_i[(tile_id(da, I - 1, J, Kc) * TK + fq) * 8 + fe] : 0.f;
+    sh.fj[fq][fe] = face_j ? da.face_j[(tile_id(da, I, J - 1, Kc) * TK + fq) * 8 + fe] : 0.f;
+  }
+  if constexpr (!kStaged) {
+    for (int q = (int)threadIdx.x; q < nk; q += 64 * W) {
+      sh.dph[q] = m.DPH[k0 + q];
+      sh.dpx[q] = m.DPL[k0 + q];
+    }
   }
   __syncthreads();
   if (wave == 0) {
@@ -696,7 +716,7 @@ __global__ void __launch_bounds__(8 * TK)
     const float t_init = t_own;
     const int S = nk + SKEW;
     auto march = [&](auto seam) {
-      constexpr bool W = decltype(seam)::value;
+      constexpr bool kSeam = decltype(seam)::value;
       float t_k = t_init, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
@@ -705,9 +725,16 @@ __global__ void __launch_bounds__(8 * TK)
           const int qs = min(max(s0 + u - ii - jj, 0), nk - 1);
           rr[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
-          ak[u] = sh.ck[qs][lane];
-          aj[u] = sh.cj[qs][lane];
-          ai[u] = sh.ci[qs][lane];
+          if constexpr (kStaged) {
+            ak[u] = sh.ck[qs][lane];
+            aj[u] = sh.cj[qs][lane];
+            ai[u] = sh.ci[qs][lane];
+          } else {
+            const double dph = sh.dph[qs];
+            ak[u] = (float)(-(P1 * sh.dpx[qs]));
+            aj[u] = (float)(-((DRH * dph) * CTL));
+            ai[u] = (float)(-((SD * dph) * CRL));
+          }
           fl[u] = sh.fi[qs][jj];
           fd[u] = sh.fj[qs][ii];
         }
@@ -717,7 +744,7 @@ __global__ void __launch_bounds__(8 * TK)
           float from_i, from_j;
           lanes_from_lower(t_k, from_i, from_j);
           float y = rr[u];
-          if constexpr (W) {
+          if constexpr (kSeam) {
             if (q == nk - 1) y = y - aw * tw;
           }
           const float tk = (q > 0) ? t_k : t_init;
@@ -726,7 +753,7 @@ __global__ void __launch_bounds__(8 * TK)
           if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
           sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = y;
           const float tt = dd[u] * y;
-          if constexpr (W) {
+          if constexpr (kSeam) {
             if (k0 == 0 && q == 0) tw = tt;
           }
           t_k = tt;
@@ -741,24 +768,28 @@ __global__ void __launch_bounds__(8 * TK)
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
+    for (int u = 0; u < kQPW; u++) {
+      const int q = wave * kQPW + u;
       if (q < nk) da.yv_t[tb + (size_t)q * 64] = sh.v[q][lane];
     }
   }
   // outgoing faces t = dinv*y of the last column (ii = 7) and row (jj = 7)
-  if (fq < nk) {
-    const size_t tb = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
-    da.face_i[tb] = sh.d[fq][TI - 1 + TI * fe] * sh.v[fq][TI - 1 + TI * fe];
-    da.face_j[tb] = sh.d[fq][fe + TI * (TJ - 1)] * sh.v[fq][fe + TI * (TJ - 1)];
+  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
+    const int fq = f >> 3, fe = f & 7;
+    const size_t tf = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
+    da.face_i[tf] = sh.d[fq][TI - 1 + TI * fe] * sh.v[fq][TI - 1 + TI * fe];
+    da.face_j[tf] = sh.d[fq][fe + TI * (TJ - 1)] * sh.v[fq][fe + TI * (TJ - 1)];
   }
 }
 
-template <typename T, int TK>
-__global__ void __launch_bounds__(8 * TK)
+template <typename T, int TK, int W>
+__global__ void __launch_bounds__(64 * W)
     k_tile_bwd_lds(Metric m, Grid3 g, DiagArgs da, T* __restrict__ z, const CgState* guard) {
+  constexpr bool kStaged = W > 1;
+  constexpr int kQPW = TK / W;
+  constexpr int kRound = kQPW < 16 ? kQPW : 16;
   const int done = guard ? st_done(guard) : 0;
-  __shared__ TileLds<TK> sh;
+  __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
   const int k0 = Kc * TK, k1 = min(g.c, k0 + TK), nk = k1 - k0;
@@ -777,26 +808,18 @@ __global__ void __launch_bounds__(8 * TK)
   const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
   float t_own = 0.f, t_w = 0.f;
   if (wave == 0) {
-    // plane k1: the first plane of the tile above; plane c-1: in the last tile
-    if (act && hi_k1) t_own = da.yv_t[tile_id(da, I, J, Kc + 1) * kTile + lane];
+    // plane k1 (the first plane of the tile above) and plane c-1 come from
+    // the compact k faces those tiles left
+    if (act && hi_k1) t_own = da.face_k[tile_id(da, I, J, Kc + 1) * 64 + lane];
     if (act && wrap && k1 < g.c && k0 == 0)
-      t_w = da.yv_t[tile_id(da, I, J, da.NK - 1) * kTile +
-                    (size_t)((g.c - 1) - (da.NK - 1) * TK) * 64 + lane];
+      t_w = da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane];
   }
-  // incoming faces (z of the first column / row of the upper neighbours)
-  const int fq = threadIdx.x >> 3, fe = threadIdx.x & 7;
-  float fi_s = 0.f, fj_s = 0.f;
-  if (fq < nk) {
-    if (I + 1 < da.NI && da.bl.hi_i(I * TI + TI - 1, g.a))
-      fi_s = da.face_i[(tile_id(da, I + 1, J, Kc) * TK + fq) * 8 + fe];
-    if (J + 1 < da.NJ && da.bl.hi_j(J * TJ + TJ - 1, g.b))
-      fj_s = da.face_j[(tile_id(da, I, J + 1, Kc) * TK + fq) * 8 + fe];
-  }
-  {
-    float yv8[8], dv[8];
+#pragma unroll 1
+  for (int q0 = wave * kQPW; q0 < (wave + 1) * kQPW; q0 += kRound) {
+    float yv8[kRound], dv[kRound];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
+    for (int u = 0; u < kRound; u++) {
+      const int q = q0 + u;
       yv8[u] = dv[u] = 0.f;
       if (act && q < nk) {
         yv8[u] = da.yv_t[tb + (size_t)q * 64];
@@ -805,21 +828,33 @@ __global__ void __launch_bounds__(8 * TK)
     }
     if (done) return;  // uniform over the block
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
+    for (int u = 0; u < kRound; u++) {
+      const int q = q0 + u;
       if (q < nk) {
-        const double dph = m.DPH[k0 + q];
         sh.v[q][lane] = yv8[u];
         sh.d[q][lane] = dv[u];
-        sh.ck[q][lane] = (float)(-(P1 * m.DPU[k0 + q]));
-        sh.cj[q][lane] = (float)(-((DRH * dph) * CTH));
-        sh.ci[q][lane] = (float)(-((SD * dph) * CRH));
+        if constexpr (kStaged) {
+          const double dph = m.DPH[k0 + q];
+          sh.ck[q][lane] = (float)(-(P1 * m.DPU[k0 + q]));
+          sh.cj[q][lane] = (float)(-((DRH * dph) * CTH));
+          sh.ci[q][lane] = (float)(-((SD * dph) * CRH));
+        }
       }
     }
   }
-  if (fq < nk) {
-    sh.fi[fq][fe] = fi_s;
-    sh.fj[fq][fe] = fj_s;
+  // incoming faces (z of the first column / row of the upper neighbours)
+  const bool face_i = I + 1 < da.NI && da.bl.hi_i(I * TI + TI - 1, g.a);
+  const bool face_j = J + 1 < da.NJ && da.bl.hi_j(J * TJ + TJ - 1, g.b);
+  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
+    const int fq = f >> 3, fe = f & 7;
+    sh.fi[fq][fe] = face_i ? da.face_i[(tile_id(da, I + 1, J, Kc) * TK + fq) * 8 + fe] : 0.f;
+    sh.fj[fq][fe] = face_j ? da.face_j[(tile_id(da, I, J + 1, Kc) * TK + fq) * 8 + fe] : 0.f;
+  }
+  if constexpr (!kStaged) {
+    for (int q = (int)threadIdx.x; q < nk; q += 64 * W) {
+      sh.dph[q] = m.DPH[k0 + q];
+      sh.dpx[q] = m.DPU[k0 + q];
+    }
   }
   __syncthreads();
   if (wave == 0) {
@@ -831,7 +866,7 @@ __global__ void __launch_bounds__(8 * TK)
     const float t_init = t_own;
     const int S = nk + SKEW;
     auto march = [&](auto seam) {
-      constexpr bool W = decltype(seam)::value;
+      constexpr bool kSeam = decltype(seam)::value;
       float t_k = t_init, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
@@ -840,9 +875,16 @@ __global__ void __launch_bounds__(8 * TK)
           const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), 0), nk - 1);
           yy[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
-          bk[u] = sh.ck[qs][lane];
-          bj[u] = sh.cj[qs][lane];
-          bi[u] = sh.ci[qs][lane];
+          if constexpr (kStaged) {
+            bk[u] = sh.ck[qs][lane];
+            bj[u] = sh.cj[qs][lane];
+            bi[u] = sh.ci[qs][lane];
+          } else {
+            const double dph = sh.dph[qs];
+            bk[u] = (float)(-(P1 * sh.dpx[qs]));
+            bj[u] = (float)(-((DRH * dph) * CTH));
+            bi[u] = (float)(-((SD * dph) * CRH));
+          }
           fr[u] = sh.fi[qs][jj];
           fu[u] = sh.fj[qs][ii];
         }
@@ -856,12 +898,12 @@ __global__ void __launch_bounds__(8 * TK)
           if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
           const float tk = (q < nk - 1) ? t_k : t_init;
           if (q < nk - 1 || hi_k1) sv = sv - bk[u] * tk;
-          if constexpr (W) {
+          if constexpr (kSeam) {
             if (q == 0) sv = sv - bw * tw;
           }
           const float zz = sv * dd[u];
           sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = zz;
-          if constexpr (W) {
+          if constexpr (kSeam) {
             if (k1 == g.c && q == nk - 1) tw = zz;
           }
           t_k = zz;
@@ -876,20 +918,34 @@ __global__ void __launch_bounds__(8 * TK)
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = 8 * wave + u;
-      if (q < nk) {
-        const float zz = sh.v[q][lane];
-        da.yv_t[tb + (size_t)q * 64] = zz;
-        z[tb + (size_t)q * 64] = (T)zz;
-      }
+    for (int u = 0; u < kQPW; u++) {
+      const int q = wave * kQPW + u;
+      if (q < nk) z[tb + (size_t)q * 64] = (T)sh.v[q][lane];
+    }
+    if (wave == 0) {  // k faces: the first plane, and plane c-1 for the seam
+      da.face_k[tile_id(da, I, J, Kc) * 64 + lane] = sh.v[0][lane];
+      if (k1 == g.c) da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane] = sh.v[nk - 1][lane];
     }
   }
   // outgoing faces: z of the first column (ii = 0) and row (jj = 0)
-  if (fq < nk) {
-    const size_t tb = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
-    da.face_i[tb] = sh.v[fq][TI * fe];
-    da.face_j[tb] = sh.v[fq][fe];
+  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
+    const int fq = f >> 3, fe = f & 7;
+    const size_t tf = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
+    da.face_i[tf] = sh.v[fq][TI * fe];
+    da.face_j[tf] = sh.v[fq][fe];
+  }
+}
+
+template <typename T, int TK, int W>
+void tiles_launch_lds_w(const SweepPlan& plan, const Metric& m, const Grid3& g, const T* r, T* z,
+                        const CgState* guard, hipStream_t s) {
+  for (int L = 0; L < plan.ndiag; L++) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    k_tile_fwd_lds<T, TK, W><<<cnt, 64 * W, 0, s>>>(m, g, diag_args(plan, L), r, guard);
+  }
+  for (int L = plan.ndiag - 1; L >= 0; L--) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    k_tile_bwd_lds<T, TK, W><<<cnt, 64 * W, 0, s>>>(m, g, diag_args(plan, L), z, guard);
   }
 }
 
@@ -899,14 +955,10 @@ void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          hipStream_t s) {
   (void)dinv;
   (void)yv;  // the single-wave sweeps use the plan's tile-major copies
-  for (int L = 0; L < plan.ndiag; L++) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_fwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L), r, guard);
-  }
-  for (int L = plan.ndiag - 1; L >= 0; L--) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    k_tile_bwd_lds<T, TK><<<cnt, 8 * TK, 0, s>>>(m, g, diag_args(plan, L), z, guard);
-  }
+  if (plan.stage_one)
+    tiles_launch_lds_w<T, TK, 1>(plan, m, g, r, z, guard, s);
+  else
+    tiles_launch_lds_w<T, TK, TK / 8>(plan, m, g, r, z, guard, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1234,7 +1286,7 @@ __global__ void k_tile_gather(Grid3 g, int TNI, int TNJ, int TK, size_t n,
 }
 
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves,
-                     const IluBlocks* blk, const float* dinv) {
+                     const IluBlocks* blk, const float* dinv, int stage_one) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   plan.NI = (g.a + TI - 1) / TI;
@@ -1274,6 +1326,7 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     return 1;
   }
   // tile diagonals
+  plan.stage_one = stage_one ? 1 : 0;
   plan.TK = (tile_k == 16 || tile_k == 64) ? tile_k : kTileK;
   plan.SI = plan.SJ = 1;
   if (tile_waves == 4) plan.SI = plan.SJ = 2;
@@ -1328,8 +1381,12 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     // tile-major dinv and y of the single-wave sweeps: every tile reads and
     // writes whole 256-byte planes instead of 32-byte row pieces
     plan.tile_elems = (size_t)plan.TNI * plan.TNJ * plan.NK * plan.TK * 64;
+    const size_t ntile = (size_t)plan.TNI * plan.TNJ * plan.NK;
     if (hipMalloc((void**)&plan.dinv_t, plan.tile_elems * sizeof(float)) != hipSuccess ||
-        hipMalloc((void**)&plan.yv_t, plan.tile_elems * sizeof(float)) != hipSuccess) {
+        hipMalloc((void**)&plan.yv_t, plan.tile_elems * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&plan.face_k, ntile * 64 * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&plan.face_kt, (size_t)plan.TNI * plan.TNJ * 64 * sizeof(float)) !=
+            hipSuccess) {
       free_sweep_plan(plan);
       return 1;
     }
@@ -1360,6 +1417,8 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.face_j) hipFree(plan.face_j);
   if (plan.dinv_t) hipFree(plan.dinv_t);
   if (plan.yv_t) hipFree(plan.yv_t);
+  if (plan.face_k) hipFree(plan.face_k);
+  if (plan.face_kt) hipFree(plan.face_kt);
   plan = SweepPlan();
 }
 

@@ -55,6 +55,11 @@ struct SweepPlan {
   size_t tile_elems = 0;
   float* dinv_t = nullptr;
   float* yv_t = nullptr;
+  float* face_k = nullptr;   // per tile: backward z of its first plane
+  float* face_kt = nullptr;  // per column of tiles: backward z of plane c-1
+  // single-wave tiles: 1 when the marching wave stages its tile alone (one
+  // wave and about 20 KB of LDS per tile), 0 when TK/8 waves stage it
+  int stage_one = 0;
 };
 
 // Returns 0 when the dataflow sweep can run on this grid.
@@ -64,9 +69,10 @@ struct SweepPlan {
 // multiples of the tile extents (returns nonzero otherwise).
 // dinv: the D-ILU's inverse pivots (natural order), copied tile-major for
 // the single-wave tile sweeps (build the plan again after refactoring).
+// stage_one: see SweepPlan::stage_one.
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32,
                      int tile_waves = 1, const IluBlocks* bl = nullptr,
-                     const float* dinv = nullptr);
+                     const float* dinv = nullptr, int stage_one = 0);
 void free_sweep_plan(SweepPlan& plan);
 // 1 if a sweep wait exceeded its bound since the last call (and clears it).
 int sweep_error(const SweepPlan& plan);

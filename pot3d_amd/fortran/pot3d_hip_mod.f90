@@ -21,7 +21,8 @@ module pot3d_hip
                                    P3D_OPT_TILE_K=9, P3D_OPT_TILE_WAVES=10, &
                                    P3D_OPT_ILU_BLOCKS_R=11,             &
                                    P3D_OPT_ILU_BLOCKS_T=12,             &
-                                   P3D_OPT_ILU_BLOCKS_P=13
+                                   P3D_OPT_ILU_BLOCKS_P=13,             &
+                                   P3D_OPT_TILE_STAGE=14
 !
       interface
 ! ****** Whole-solve ABI.

@@ -134,15 +134,20 @@ def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
             assert np.array_equal(z_df, z_df2)
             assert np.array_equal(z_tl, z_lv), (rep, np.max(np.abs(z_tl - z_lv)))
             assert np.array_equal(z_tl, z_tl2)
-            # other tile shapes: waves per tile x planes per tile
-            for waves, tk in ((1, 16), (1, 64), (4, 16), (4, 32), (4, 64), (8, 16), (8, 32)):
+            # other tile shapes: waves per tile x planes per tile, and
+            # single-wave tiles staged by the marching wave alone
+            for waves, tk, stage in ((1, 16, 0), (1, 64, 0), (4, 16, 0), (4, 32, 0),
+                                     (4, 64, 0), (8, 16, 0), (8, 32, 0), (1, 16, 1),
+                                     (1, 32, 1), (1, 64, 1)):
                 s.set_option(_lib.OPT_TILE_WAVES, waves)
                 s.set_option(_lib.OPT_TILE_K, tk)
+                s.set_option(_lib.OPT_TILE_STAGE, stage)
                 z_tk = s.prec_inv(x)
-                assert np.array_equal(z_tk, z_lv), (rep, waves, tk,
+                assert np.array_equal(z_tk, z_lv), (rep, waves, tk, stage,
                                                     np.max(np.abs(z_tk - z_lv)))
             s.set_option(_lib.OPT_TILE_WAVES, 1)
             s.set_option(_lib.OPT_TILE_K, 32)
+            s.set_option(_lib.OPT_TILE_STAGE, 0)
 
 
 @pytest.mark.skip(reason="the persistent dataflow sweep (P3D_OPT_SWEEP=2) inside the CG "
@@ -392,15 +397,18 @@ def test_block_jacobi_ilu(oracle_mod, solver_mod, nblk):
         s.set_option(_lib.OPT_SWEEP, 1)
         z_lv = s.prec_inv(x)
         s.set_option(_lib.OPT_SWEEP, 0)
-        for waves, tk in ((1, 32), (1, 64), (4, 64), (8, 32)):
+        for waves, tk, stage in ((1, 32, 0), (1, 64, 0), (4, 64, 0), (8, 32, 0), (1, 16, 1),
+                                 (1, 32, 1)):
             s.set_option(_lib.OPT_TILE_WAVES, waves)
             s.set_option(_lib.OPT_TILE_K, tk)
+            s.set_option(_lib.OPT_TILE_STAGE, stage)
             z = s.prec_inv(x)
-            assert np.array_equal(z, z_lv), (waves, tk, np.max(np.abs(z - z_lv)))
+            assert np.array_equal(z, z_lv), (waves, tk, stage, np.max(np.abs(z - z_lv)))
         zref = o.dilu_apply(x.astype(np.float32), blocks=ext).astype(np.float64)
         assert np.max(np.abs(z_lv - zref)) <= 2e-5 * np.max(np.abs(zref))
         s.set_option(_lib.OPT_TILE_WAVES, 1)
         s.set_option(_lib.OPT_TILE_K, 32)
+        s.set_option(_lib.OPT_TILE_STAGE, 0)
         s.set_br0(br0)
         s.set_option(_lib.OPT_STOP, 1)
         xs = np.zeros(s.N)
