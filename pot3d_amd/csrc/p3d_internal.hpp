@@ -175,8 +175,8 @@ __device__ __forceinline__ double st_load(const double* f) {
 
 // Reduction slots used by the "last block finishes" pattern.
 struct Reduce {
-  double* partial;   // [3 * max_blocks]
-  unsigned int* counter;  // one counter per reduction site
+  double* partial;   // [3 * max_blocks + 3 * 256]: block, then group partials
+  unsigned int* counter;  // 256 counters per reduction site (two-level reduction)
   int max_blocks;
 };
 

@@ -45,15 +45,38 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block reduction of NV doubles; every block stores its partial, the last
-// block to arrive sums all partials in a fixed order (deterministic for a
-// given launch shape), runs the epilogue and re-arms the counter.
+// Block reduction of NV doubles, in two levels so that no single counter
+// takes an atomic from every workgroup (thousands of same-address atomics
+// serialise at the L2 and cost hundreds of microseconds per reduction).
+// Every block stores its partial and counts in at its group's counter
+// (kRedGroup consecutive blocks); the last block of a group sums the
+// group's partials in order, stores the group partial and counts in at the
+// slot's global counter; the last group sums the group partials in order,
+// runs the epilogue and re-arms the counters.  The result is deterministic
+// for a given launch shape.
+//   counter: the slot's kRedCounters counters ([0] global, [1 + group]);
+//   partial: NV*G block partials, then NV*ngroups group partials.
+// Counts a block in at a reduction counter after its partial sums (agent-
+// scope atomic stores, coherent by themselves) have completed.  A release
+// ordering on the counter would make the compiler write the whole L2 back
+// (buffer_wbl2) in every block, including the kernel's own output lines;
+// waiting for the block's stores to complete is all the protocol needs.
+__device__ __forceinline__ unsigned arrive(unsigned* counter) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int NV>
 __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter, int G, int Lb,
                              CgState* st, Epi epi, double* out) {
   __shared__ double sh[NV][kBlock / 64];
   __shared__ int amlast;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int grp = Lb / kRedGroup, g0 = grp * kRedGroup;
+  const int gn = min(kRedGroup, G - g0), ngrp = (G + kRedGroup - 1) / kRedGroup;
+  double* gpart = partial + (size_t)NV * G;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     const double t = wave_sum(v[q]);
@@ -67,19 +90,19 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
       for (int ww = 0; ww < kBlock / 64; ww++) s += sh[q][ww];
       __hip_atomic_store(&partial[q * G + Lb], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    const unsigned prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    amlast = (prev == (unsigned)G - 1);
+    const unsigned prev = arrive(counter + 1 + grp);
+    amlast = (prev == (unsigned)gn - 1);
   }
   __syncthreads();
   if (!amlast) return;
   __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  double tot[NV];
+  // last block of the group: the group partial
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double s = 0.0;
-    for (int m = threadIdx.x; m < G; m += kBlock)
-      s += __hip_atomic_load(&partial[q * G + m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int m = threadIdx.x; m < gn; m += kBlock)
+      s += __hip_atomic_load(&partial[q * G + g0 + m], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
     s = wave_sum(s);
     if (lane == 0) sh[q][w] = s;
   }
@@ -89,19 +112,34 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
     for (int q = 0; q < NV; q++) {
       double s = 0.0;
       for (int ww = 0; ww < kBlock / 64; ww++) s += sh[q][ww];
-      tot[q] = s;
+      __hip_atomic_store(&gpart[q * ngrp + grp], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (NV == 3) {
-      if (out) {
-        out[0] = tot[0];
-        out[1] = tot[NV > 1 ? 1 : 0];
-        out[2] = tot[NV > 2 ? 2 : 0];
-      }
-    } else {
-      cg_epilogue(st, epi, tot[0], NV > 1 ? tot[NV > 1 ? 1 : 0] : 0.0, out);
-    }
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(counter + 1 + grp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev = arrive(counter);
+    amlast = (prev == (unsigned)ngrp - 1);
   }
+  __syncthreads();
+  if (!amlast || threadIdx.x != 0) return;
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  // last group: the total, in group order
+  double tot[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double s = 0.0;
+    for (int gi = 0; gi < ngrp; gi++)
+      s += __hip_atomic_load(&gpart[q * ngrp + gi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot[q] = s;
+  }
+  if (NV == 3) {
+    if (out) {
+      out[0] = tot[0];
+      out[1] = tot[NV > 1 ? 1 : 0];
+      out[2] = tot[NV > 2 ? 2 : 0];
+    }
+  } else {
+    cg_epilogue(st, epi, tot[0], NV > 1 ? tot[NV > 1 ? 1 : 0] : 0.0, out);
+  }
+  __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,7 +364,7 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
   const int nseg = stencil_segments(g);
   const int G = stencil_blocks(g);
   k_stencil_march<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
-                                           red.partial, red.counter + slot, epi, out,
+                                           red.partial, red.counter + (size_t)slot * kRedCounters, epi, out,
                                            guard ? 1 : 0, want);
 }
 
@@ -420,7 +458,7 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
                       hipStream_t s, double* xs, const double* xscale, const TileMap& tm) {
   k_cg_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, x, r, p, ap, st, red.partial,
-                                         red.counter + slot, xs, xscale, tm);
+                                         red.counter + (size_t)slot * kRedCounters, xs, xscale, tm);
 }
 
 // p = beta p + z  (:4361-4363)
@@ -506,7 +544,7 @@ void launch_dot2(const Grid3& g, const Launch& L, const T* x, const T* y, const 
                  const T* v, CgState* st, Reduce red, int slot, Epi epi, double* out,
                  bool guard, hipStream_t s, const TileMap& tm) {
   k_dot2<T><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, x, y, u, v, st, red.partial,
-                                    red.counter + slot, epi, out, guard ? 1 : 0, tm);
+                                    red.counter + (size_t)slot * kRedCounters, epi, out, guard ? 1 : 0, tm);
 }
 
 // z = M^-1 r (DIAG/NONE), reductions s0=r.z, s1=r.r.
@@ -536,7 +574,7 @@ void launch_point_prec(const Metric& m, const Grid3& g, const Launch& L, int pre
                        T* z, CgState* st, Reduce red, int slot, Epi epi, double* out,
                        hipStream_t s) {
   k_point_prec<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, r, z, st, red.partial,
-                                          red.counter + slot, epi, out);
+                                          red.counter + (size_t)slot * kRedCounters, epi, out);
 }
 
 template <typename T>

@@ -92,6 +92,11 @@ struct Launch {
 };
 
 constexpr int kBlock = 256;
+// two-level reductions (block_finish): blocks per group, counters per slot
+// ([0] global + one per group), so at most kRedGroup*(kRedCounters-1) blocks
+constexpr int kRedGroup = 128;
+constexpr int kRedCounters = 256;
+constexpr int kRedSlots = 16;
 constexpr int kUnroll = 4;  // points per thread in flight in the streaming kernels
 constexpr int kPoleSegs = 16;
 

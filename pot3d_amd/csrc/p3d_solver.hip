@@ -751,10 +751,14 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     return bail(fail(P3D_ENOMEM, "hipHostMalloc failed"));
   if ((rc = dalloc((void**)&s->out, 8 * sizeof(double)))) return bail(rc);
   s->red.max_blocks = s->L.G > stencil_blocks(s->g) ? s->L.G : stencil_blocks(s->g);
-  if ((rc = dalloc((void**)&s->red.partial, sizeof(double) * 3 * s->red.max_blocks)))
+  if (s->red.max_blocks > kRedGroup * (kRedCounters - 1))
+    return bail(fail(P3D_EINVAL, "grid too large for the reduction counters"));
+  if ((rc = dalloc((void**)&s->red.partial,
+                   sizeof(double) * 3 * (s->red.max_blocks + kRedCounters))))
     return bail(rc);
-  if ((rc = dalloc((void**)&s->red.counter, sizeof(unsigned) * 16))) return bail(rc);
-  hipMemset(s->red.counter, 0, sizeof(unsigned) * 16);
+  if ((rc = dalloc((void**)&s->red.counter, sizeof(unsigned) * kRedSlots * kRedCounters)))
+    return bail(rc);
+  hipMemset(s->red.counter, 0, sizeof(unsigned) * kRedSlots * kRedCounters);
   hipMemset(s->x, 0, nb);
   hipMemset(s->b, 0, nb);
   hipMemset(s->pole_part, 0, sizeof(double) * 2 * kPoleSegs * a);
@@ -1041,9 +1045,11 @@ int p3d_energy(p3d_solver* s, const double* phi, double* w, double* br, double* 
                           s->stream));
   if (!s->e_partial) {
     s->eG = s->L.G;
-    if ((rc = dalloc((void**)&s->e_partial, sizeof(double) * 3 * s->eG))) return rc;
+    if ((rc = dalloc((void**)&s->e_partial, sizeof(double) * 3 * (s->eG + kRedCounters))))
+      return rc;
   }
-  launch_energy(s->em, nr, nt, np, s->phi, s->e_partial, s->red.counter + 8, s->out, s->eG,
+  launch_energy(s->em, nr, nt, np, s->phi, s->e_partial, s->red.counter + 8 * kRedCounters,
+                s->out, s->eG,
                 s->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s->out_host, s->out, 3 * sizeof(double), hipMemcpyDeviceToHost,
