@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(kBlock)
     k_stencil_march(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
                     const T* __restrict__ b, int mode, const double* __restrict__ pole_part,
                     CgState* st, double* partial, unsigned* counter, Epi epi, double* out,
-                    int guard, int want_dot) {
+                    int guard, int want_dot, TileMap tm) {
   if (guard && st_done(st)) return;
   const int G = gridDim.x;
   const int Lb = logical_block(G);
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kBlock)
             if (mode == 1) yd = (double)bv[u] - yd;
             yv = (T)yd;
           }
-          y[n] = yv;
+          y[tmap(tm, g, n)] = yv;
           acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
           xkm = xk;
           xk = xkp;
@@ -358,14 +358,15 @@ int stencil_blocks(const Grid3& g) {
 template <typename T>
 void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x, T* y,
                     const T* b, int mode, const double* pole_part, CgState* st, Reduce red,
-                    int slot, Epi epi, double* out, bool guard, hipStream_t s) {
+                    int slot, Epi epi, double* out, bool guard, hipStream_t s,
+                    const TileMap& tm) {
   (void)L;
   const int want = (epi != EPI_NONE);
   const int nseg = stencil_segments(g);
   const int G = stencil_blocks(g);
   k_stencil_march<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
-                                           red.partial, red.counter + (size_t)slot * kRedCounters, epi, out,
-                                           guard ? 1 : 0, want);
+                                           red.partial, red.counter + (size_t)slot * kRedCounters,
+                                           epi, out, guard ? 1 : 0, want, tm);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +460,84 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
                       hipStream_t s, double* xs, const double* xscale, const TileMap& tm) {
   k_cg_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, x, r, p, ap, st, red.partial,
                                          red.counter + (size_t)slot * kRedCounters, xs, xscale, tm);
+}
+
+// The CG update of the FP32 recurrence with the FP64 solution, as two
+// contiguous passes: r -= alpha ap over the tile-major r and ap (r.r, EPI_RR;
+// entries outside the grid are zero on both), and xs += (alpha scale) p in
+// natural order.  (k_cg_update would read r through the tile map.)
+__global__ void __launch_bounds__(kBlock)
+    k_r_update(uint32_t n_all, int G, int CH, float* __restrict__ r, const float* __restrict__ ap,
+               CgState* st, double* partial, unsigned* counter) {
+  if (st_done(st)) return;
+  const int Lb = logical_block(G);
+  const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
+  const uint32_t n1 = min(n_all, n0 + (uint32_t)CH);
+  const float alpha = (float)st_load(&st->alpha);
+  double s1 = 0.0;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
+    float rv[kUnroll], av[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      rv[u] = av[u] = 0.f;
+      if (n < n1) {
+        rv[u] = r[n];
+        av[u] = ap[n];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      if (n < n1) {
+        const float rn = rv[u] - alpha * av[u];
+        r[n] = rn;
+        s1 += (double)rn * (double)rn;
+      }
+    }
+  }
+  double v[2] = {0.0, s1};
+  block_finish<2>(v, partial, counter, G, Lb, st, EPI_RR, nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_x_update(uint32_t N, int G, int CH, double* __restrict__ xs, const float* __restrict__ p,
+               const CgState* st, const double* __restrict__ xscale) {
+  if (st_done(st)) return;
+  const int Lb = logical_block(G);
+  const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
+  const uint32_t n1 = min(N, n0 + (uint32_t)CH);
+  const double ax = st_load(&st->alpha) * *xscale;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
+    double xv[kUnroll];
+    float pv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      xv[u] = 0.0;
+      pv[u] = 0.f;
+      if (n < n1) {
+        xv[u] = xs[n];
+        pv[u] = p[n];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t n = nb + u * kBlock;
+      if (n < n1) xs[n] = xv[u] + ax * (double)pv[u];
+    }
+  }
+}
+
+void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
+                     Reduce red, int slot, hipStream_t s) {
+  k_r_update<<<Lt.G, kBlock, 0, s>>>(n_all, Lt.G, Lt.CH, r, ap, st, red.partial,
+                                      red.counter + (size_t)slot * kRedCounters);
+}
+
+void launch_x_update(const Grid3& g, const Launch& L, double* xs, const float* p,
+                     const CgState* st, const double* xscale, hipStream_t s) {
+  k_x_update<<<L.G, kBlock, 0, s>>>(g.N, L.G, L.CH, xs, p, st, xscale);
 }
 
 // p = beta p + z  (:4361-4363)
@@ -1045,7 +1124,7 @@ void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi
                                     hipStream_t);                                               \
   template void launch_stencil<T>(const Metric&, const Grid3&, const Launch&, const T*, T*,     \
                                   const T*, int, const double*, CgState*, Reduce, int, Epi,     \
-                                  double*, bool, hipStream_t);                                  \
+                                  double*, bool, hipStream_t, const TileMap&);                  \
   template void launch_cg_update<T>(const Metric&, const Grid3&, const Launch&, int, T*, T*,    \
                                     const T*, const T*, CgState*, Reduce, int, hipStream_t,     \
                                     double*, const double*, const TileMap&);                    \

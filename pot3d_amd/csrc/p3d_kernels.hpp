@@ -112,10 +112,20 @@ int stencil_blocks(const Grid3& g);
 
 // y = A x (homogeneous boundary values; mode 0) or y = b - A x (mode 1).
 // Reduces s0 = x.y (mode 0) or s0 = y.y (mode 1) and runs epilogue `epi`.
+// tm places y (scatter; natural by default).
 template <typename T>
 void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x, T* y,
                     const T* b, int mode, const double* pole_part, CgState* st, Reduce red,
-                    int slot, Epi epi, double* out, bool guard, hipStream_t s);
+                    int slot, Epi epi, double* out, bool guard, hipStream_t s,
+                    const TileMap& tm = TileMap{});
+
+// CG update of the FP32 recurrence with an FP64 solution, as two contiguous
+// passes: r -= alpha ap over n_all tile-major entries (r.r, EPI_RR; Lt is
+// the launch shape of n_all), and xs += (alpha * *xscale) p in natural order.
+void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
+                     Reduce red, int slot, hipStream_t s);
+void launch_x_update(const Grid3& g, const Launch& L, double* xs, const float* p,
+                     const CgState* st, const double* xscale, hipStream_t s);
 
 // CG update x += alpha p, r -= alpha ap, and for PREC_DIAG/NONE the reduction
 // of r.z and r.r with epilogue EPI_RITER; for PREC_ILU only r.r (EPI_RR).

@@ -81,6 +81,8 @@ struct p3d_solver {
   // sweeps they are tile-major (tm.on), every other vector is natural
   size_t nvec = 0;
   TileMap tm;
+  Launch Lt;  // launch shape of a pass over the tile-major entries (tm.on)
+  Grid3 gt;   // g with N = the tile-major entries (contiguous FP32 passes)
   double* pole_part = nullptr;
   CgState* st = nullptr;
   CgState* st_host = nullptr;  // pinned
@@ -129,11 +131,12 @@ static int ensure_mixed(p3d_solver* s) {
   const size_t nb = (size_t)s->g.N * sizeof(float), nv = s->nvec * sizeof(float);
   int rc;
   if ((rc = dalloc((void**)&s->x32, nb)) || (rc = dalloc((void**)&s->r32, nv)) ||
-      (rc = dalloc((void**)&s->p32, nb)) || (rc = dalloc((void**)&s->ap32, nb)) ||
+      (rc = dalloc((void**)&s->p32, nb)) || (rc = dalloc((void**)&s->ap32, nv)) ||
       (rc = dalloc((void**)&s->z32, nv)))
     return rc;
   HIPCHK(hipMemsetAsync(s->r32, 0, nv, s->stream));
   HIPCHK(hipMemsetAsync(s->z32, 0, nv, s->stream));
+  HIPCHK(hipMemsetAsync(s->ap32, 0, nv, s->stream));
   return P3D_OK;
 }
 
@@ -157,10 +160,20 @@ static int update_tilemap(p3d_solver* s) {
     s->gexec[q] = nullptr;
   }
   s->tm = t;
+  if (t.on) {
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    s->gt = s->g;
+    s->gt.N = (uint32_t)s->sweep.tile_elems;
+    // as many blocks as the reduction buffers hold (a multiple of 8)
+    s->Lt = plan_stream(s->gt.N, ncu * 8 < s->red.max_blocks ? ncu * 8 : s->red.max_blocks);
+  }
   HIPCHK(hipMemsetAsync(s->r, 0, s->nvec * sizeof(double), s->stream));
   HIPCHK(hipMemsetAsync(s->z, 0, s->nvec * sizeof(double), s->stream));
   if (s->r32) HIPCHK(hipMemsetAsync(s->r32, 0, s->nvec * sizeof(float), s->stream));
   if (s->z32) HIPCHK(hipMemsetAsync(s->z32, 0, s->nvec * sizeof(float), s->stream));
+  if (s->ap32) HIPCHK(hipMemsetAsync(s->ap32, 0, s->nvec * sizeof(float), s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return P3D_OK;
 }
@@ -201,10 +214,14 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
       launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, s->blocks, gd,
                                   s->stream);
     }
-    if (epi != EPI_NONE)
-      launch_dot2<T>(s->g, s->L, r, z, (epi == EPI_RZITER) ? nullptr : r,
-                     (epi == EPI_RZITER) ? nullptr : r, s->st, s->red, 1, epi, out, guard,
-                     s->stream, s->tm);
+    if (epi != EPI_NONE) {
+      const T* u = (epi == EPI_RZITER) ? nullptr : r;
+      if (sizeof(T) == 4 && s->tm.on)  // FP32: contiguous, in tile order
+        launch_dot2<T>(s->gt, s->Lt, r, z, u, u, s->st, s->red, 1, epi, out, guard, s->stream);
+      else  // FP64: natural point order, as every other schedule
+        launch_dot2<T>(s->g, s->L, r, z, u, u, s->st, s->red, 1, epi, out, guard, s->stream,
+                       s->tm);
+    }
   } else {
     launch_point_prec<T>(s->m, s->g, s->L, s->ifprec, r, z, s->st, s->red, 1, epi, out,
                          s->stream);
@@ -213,6 +230,20 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
 
 template <typename T>
 static void cg_iteration(p3d_solver* s, Vecs<T> v) {
+  if constexpr (sizeof(T) == 4) {
+    if (v.xs && s->tm.on) {
+      // FP32 recurrence with the FP64 solution and tile-major r, ap, z:
+      // every pass contiguous except the p update's read of z
+      launch_pole_sums<float>(s->m, s->g, v.p, s->pole_part, s->stream);
+      launch_stencil<float>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red,
+                            0, EPI_PAP, nullptr, true, s->stream, s->tm);
+      launch_r_update(s->gt.N, s->Lt, v.r, v.ap, s->st, s->red, 2, s->stream);
+      launch_x_update(s->g, s->L, v.xs, v.p, s->st, v.xscale, s->stream);
+      prec_and_dot<float>(s, v.r, v.z, EPI_RZITER, nullptr, true);
+      launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm);
+      return;
+    }
+  }
   launch_pole_sums<T>(s->m, s->g, v.p, s->pole_part, s->stream);
   launch_stencil<T>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red, 0,
                     EPI_PAP, nullptr, true, s->stream);
