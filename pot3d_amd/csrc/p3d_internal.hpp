@@ -159,7 +159,7 @@ struct CgState {
   int done;          // 0 running, 1 converged, 2 ncgmax reached,
                      // 3 non-finite residual, 4 zero right-hand side
   int stop_mode;     // 0: epsn (reference), 1: ||r||2/||b||2
-  int pad_;
+  int x_ncg;         // iterations whose FP64 solution update is applied (FP32 recurrence)
 };
 
 // Control-state reads in kernels go through agent-scope atomic loads, so
@@ -168,6 +168,9 @@ struct CgState {
 // break the last-block / last-wave protocols).
 __device__ __forceinline__ int st_done(const CgState* st) {
   return __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int st_load_int(const int* f) {
+  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double st_load(const double* f) {
   return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

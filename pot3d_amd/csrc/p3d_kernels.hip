@@ -68,10 +68,10 @@ __device__ __forceinline__ unsigned arrive(unsigned* counter) {
   return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int NV>
-__device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter, int G, int Lb,
+template <int BS, int NV>
+__device__ void block_finish_bs(double (&v)[NV], double* partial, unsigned* counter, int G, int Lb,
                              CgState* st, Epi epi, double* out) {
-  __shared__ double sh[NV][kBlock / 64];
+  __shared__ double sh[NV][BS / 64];
   __shared__ int amlast;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int grp = Lb / kRedGroup, g0 = grp * kRedGroup;
@@ -87,7 +87,7 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       double s = 0.0;
-      for (int ww = 0; ww < kBlock / 64; ww++) s += sh[q][ww];
+      for (int ww = 0; ww < BS / 64; ww++) s += sh[q][ww];
       __hip_atomic_store(&partial[q * G + Lb], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const unsigned prev = arrive(counter + 1 + grp);
@@ -100,7 +100,7 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double s = 0.0;
-    for (int m = threadIdx.x; m < gn; m += kBlock)
+    for (int m = threadIdx.x; m < gn; m += BS)
       s += __hip_atomic_load(&partial[q * G + g0 + m], __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
     s = wave_sum(s);
@@ -111,7 +111,7 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       double s = 0.0;
-      for (int ww = 0; ww < kBlock / 64; ww++) s += sh[q][ww];
+      for (int ww = 0; ww < BS / 64; ww++) s += sh[q][ww];
       __hip_atomic_store(&gpart[q * ngrp + grp], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __hip_atomic_store(counter + 1 + grp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -140,6 +140,12 @@ __device__ void block_finish(double (&v)[NV], double* partial, unsigned* counter
     cg_epilogue(st, epi, tot[0], NV > 1 ? tot[NV > 1 ? 1 : 0] : 0.0, out);
   }
   __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NV>
+__device__ __forceinline__ void block_finish(double (&v)[NV], double* partial, unsigned* counter,
+                                             int G, int Lb, CgState* st, Epi epi, double* out) {
+  block_finish_bs<kBlock, NV>(v, partial, counter, G, Lb, st, epi, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,10 +468,10 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
                                          red.counter + (size_t)slot * kRedCounters, xs, xscale, tm);
 }
 
-// The CG update of the FP32 recurrence with the FP64 solution, as two
-// contiguous passes: r -= alpha ap over the tile-major r and ap (r.r, EPI_RR;
-// entries outside the grid are zero on both), and xs += (alpha scale) p in
-// natural order.  (k_cg_update would read r through the tile map.)
+// The residual update of the FP32 recurrence with the FP64 solution, one
+// contiguous pass over the tile-major r and ap: r -= alpha ap, r.r (EPI_RR);
+// entries outside the grid are zero on both.  (k_cg_update would read r
+// through the tile map.)  The solution update goes with the p update.
 __global__ void __launch_bounds__(kBlock)
     k_r_update(uint32_t n_all, int G, int CH, float* __restrict__ r, const float* __restrict__ ap,
                CgState* st, double* partial, unsigned* counter) {
@@ -500,82 +506,80 @@ __global__ void __launch_bounds__(kBlock)
   block_finish<2>(v, partial, counter, G, Lb, st, EPI_RR, nullptr);
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_x_update(uint32_t N, int G, int CH, double* __restrict__ xs, const float* __restrict__ p,
-               const CgState* st, const double* __restrict__ xscale) {
-  if (st_done(st)) return;
-  const int Lb = logical_block(G);
-  const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
-  const uint32_t n1 = min(N, n0 + (uint32_t)CH);
-  const double ax = st_load(&st->alpha) * *xscale;
-  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
-    double xv[kUnroll];
-    float pv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const uint32_t n = nb + u * kBlock;
-      xv[u] = 0.0;
-      pv[u] = 0.f;
-      if (n < n1) {
-        xv[u] = xs[n];
-        pv[u] = p[n];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const uint32_t n = nb + u * kBlock;
-      if (n < n1) xs[n] = xv[u] + ax * (double)pv[u];
-    }
-  }
-}
-
 void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
                      Reduce red, int slot, hipStream_t s) {
   k_r_update<<<Lt.G, kBlock, 0, s>>>(n_all, Lt.G, Lt.CH, r, ap, st, red.partial,
                                       red.counter + (size_t)slot * kRedCounters);
 }
 
-void launch_x_update(const Grid3& g, const Launch& L, double* xs, const float* p,
-                     const CgState* st, const double* xscale, hipStream_t s) {
-  k_x_update<<<L.G, kBlock, 0, s>>>(g.N, L.G, L.CH, xs, p, st, xscale);
-}
-
 // p = beta p + z  (:4361-4363)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_p_update(Metric m, Grid3 g, int G, int CH, int prec, T* __restrict__ p,
-               const T* __restrict__ r, const T* __restrict__ z, CgState* st, TileMap tm) {
-  if (st_done(st)) return;
+               const T* __restrict__ r, const T* __restrict__ z, CgState* st, TileMap tm,
+               double* __restrict__ xs, const double* __restrict__ xscale,
+               unsigned* counter) {
+  // With xs (FP32 recurrence, FP64 solution): first the solution update of
+  // the iteration whose alpha is current, xs += (alpha scale) p_old, once per
+  // iteration, including the iteration that set done (CgState::x_ncg counts
+  // the applied updates; the last workgroup advances it, after every
+  // workgroup has read it); then p = z + beta p_old unless done.
+  const int done = st_done(st);
+  const int ncg = xs ? st_load_int(&st->ncg) : 0;
+  const bool do_x = xs && st_load_int(&st->x_ncg) < ncg;
+  if (done && !do_x) return;
   const int Lb = logical_block(G);
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   const T beta = (T)st_load(&st->beta);
+  const double alpha_x = do_x ? st_load(&st->alpha) * *xscale : 0.0;
   for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
     T pv[kUnroll], zv[kUnroll];
+    double xv[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t n = nb + u * kBlock;
       pv[u] = zv[u] = (T)0;
+      xv[u] = 0.0;
       if (n < n1) {
         pv[u] = p[n];
-        zv[u] = (prec == PREC_ILU) ? z[tmap(tm, g, n)] : r[n];
+        if (do_x) xv[u] = xs[n];
+        if (!done) zv[u] = (prec == PREC_ILU) ? z[tmap(tm, g, n)] : r[n];
       }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t n = nb + u * kBlock;
       if (n < n1) {
-        const T zn = (prec == PREC_ILU) ? zv[u] : apply_point_prec<T>(m, g, prec, n, zv[u]);
-        p[n] = beta * pv[u] + zn;
+        if (do_x) xs[n] = xv[u] + alpha_x * (double)pv[u];
+        if (!done) {
+          const T zn = (prec == PREC_ILU) ? zv[u] : apply_point_prec<T>(m, g, prec, n, zv[u]);
+          p[n] = beta * pv[u] + zn;
+        }
       }
+    }
+  }
+  if (do_x) {
+    __shared__ int amlast;
+    __syncthreads();
+    // only the reads of x_ncg must precede its update: a plain count-in
+    if (threadIdx.x == 0)
+      amlast = (__hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                (unsigned)G - 1);
+    __syncthreads();
+    if (amlast && threadIdx.x == 0) {
+      __hip_atomic_store(&st->x_ncg, ncg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
-                     const T* r, const T* z, CgState* st, hipStream_t s, const TileMap& tm) {
-  k_p_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, p, r, z, st, tm);
+                     const T* r, const T* z, CgState* st, hipStream_t s, const TileMap& tm,
+                     double* xs, const double* xscale, unsigned* counter) {
+  k_p_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, p, r, z, st, tm, xs, xscale,
+                                        counter);
 }
 
 // s0 = x.y, s1 = u.v
@@ -1129,7 +1133,8 @@ void launch_getb(const EnergyMesh& em, int nr, int nt, int np, const double* phi
                                     const T*, const T*, CgState*, Reduce, int, hipStream_t,     \
                                     double*, const double*, const TileMap&);                    \
   template void launch_p_update<T>(const Metric&, const Grid3&, const Launch&, int, T*,         \
-                                   const T*, const T*, CgState*, hipStream_t, const TileMap&);  \
+                                   const T*, const T*, CgState*, hipStream_t, const TileMap&,   \
+                                   double*, const double*, unsigned*);                          \
   template void launch_dot2<T>(const Grid3&, const Launch&, const T*, const T*, const T*,       \
                                const T*, CgState*, Reduce, int, Epi, double*, bool,             \
                                hipStream_t, const TileMap&);                                    \

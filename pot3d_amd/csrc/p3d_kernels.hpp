@@ -119,13 +119,10 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     int slot, Epi epi, double* out, bool guard, hipStream_t s,
                     const TileMap& tm = TileMap{});
 
-// CG update of the FP32 recurrence with an FP64 solution, as two contiguous
-// passes: r -= alpha ap over n_all tile-major entries (r.r, EPI_RR; Lt is
-// the launch shape of n_all), and xs += (alpha * *xscale) p in natural order.
+// Residual update of the FP32 recurrence: r -= alpha ap over n_all
+// tile-major entries, r.r (EPI_RR); Lt is the launch shape of n_all.
 void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
                      Reduce red, int slot, hipStream_t s);
-void launch_x_update(const Grid3& g, const Launch& L, double* xs, const float* p,
-                     const CgState* st, const double* xscale, hipStream_t s);
 
 // CG update x += alpha p, r -= alpha ap, and for PREC_DIAG/NONE the reduction
 // of r.z and r.r with epilogue EPI_RITER; for PREC_ILU only r.r (EPI_RR).
@@ -139,10 +136,14 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
 
 // p = z + beta p (z = M^-1 r computed in place for DIAG/NONE, read for ILU).
 // tm: placement of r (cg_update) and z (p_update) (natural by default).
+// With xs (FP32 recurrence, FP64 solution) it first applies the pending
+// solution update xs += (alpha * *xscale) p, once per iteration
+// (CgState::x_ncg; counter: a reduction slot's counter).
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
                      const T* r, const T* z, CgState* st, hipStream_t s,
-                     const TileMap& tm = TileMap{});
+                     const TileMap& tm = TileMap{}, double* xs = nullptr,
+                     const double* xscale = nullptr, unsigned* counter = nullptr);
 
 // Generic fused dot: s0 = x.y, s1 = u.v (v/u may be null), epilogue epi.
 // All four vectors are placed by tm; the sums run in natural point order.

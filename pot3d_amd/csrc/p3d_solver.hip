@@ -238,9 +238,10 @@ static void cg_iteration(p3d_solver* s, Vecs<T> v) {
       launch_stencil<float>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red,
                             0, EPI_PAP, nullptr, true, s->stream, s->tm);
       launch_r_update(s->gt.N, s->Lt, v.r, v.ap, s->st, s->red, 2, s->stream);
-      launch_x_update(s->g, s->L, v.xs, v.p, s->st, v.xscale, s->stream);
       prec_and_dot<float>(s, v.r, v.z, EPI_RZITER, nullptr, true);
-      launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm);
+      // the p update also applies this iteration's x += alpha scale p_old
+      launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm,
+                             v.xs, v.xscale, s->red.counter + 5 * kRedCounters);
       return;
     }
   }
