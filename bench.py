@@ -10,7 +10,7 @@ Reported:
   value        time-to-solution of the fastest path, seconds: FP32 inner
                PCG with reliable updates in FP64 defect correction, the
                block-Jacobi ILU0 (the reference's per-rank ILU0 on a grid cut
-               into 2x8x8 blocks) and the learned warm start (network trained
+               into 8x16x16 blocks) and the learned warm start (network trained
                on 51x91x181 PCG solutions, prolonged to the fine grid); the
                timed region includes the network's prediction
   fp64_plain   the same solve with the plain FP64 single-rank ILU0 PCG (the
@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="the 301x601x1201 PFCS-style problem: 'open' at r in [2.5, 10], "
                          "16 bipoles (one step, no CPU baseline by default)")
-    ap.add_argument("--blocks", type=str, default="2,8,8",
+    ap.add_argument("--blocks", type=str, default="8,16,16",
                     help="block-Jacobi ILU0 blocks (r,theta,phi) of the fast paths; "
                          "1,1,1 = single-rank ILU0")
     ap.add_argument("--tile-k", type=int, default=16,
