@@ -593,17 +593,18 @@ __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int 
 // tiles resident per CU).
 template <int TK, bool STAGED>
 struct TileLds {
-  // per plane and lane; row TK is scratch, written by lanes whose step
-  // falls outside the tile's planes (keeps the march free of branches)
-  float v[TK + 1][64];  // r (forward) or y (backward), overwritten by the result
-  float d[TK + 1][64];  // dinv
+  // per plane and lane: plane q in row q+1; rows 0 and nk+1 are scratch,
+  // read and written by lanes whose step falls outside the tile's planes
+  // (one clamped row index serves every access of a step)
+  float v[TK + 2][64];  // r (forward) or y (backward), overwritten by the result
+  float d[TK + 2][64];  // dinv
   // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
-  float ck[STAGED ? TK + 1 : 1][64];
-  float cj[STAGED ? TK + 1 : 1][64];
-  float ci[STAGED ? TK + 1 : 1][64];
-  double dph[STAGED ? 1 : TK + 1], dpx[STAGED ? 1 : TK + 1];  // DPH and DPL / DPU
-  float fi[TK + 1][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
-  float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
+  float ck[STAGED ? TK + 2 : 1][64];
+  float cj[STAGED ? TK + 2 : 1][64];
+  float ci[STAGED ? TK + 2 : 1][64];
+  double dph[STAGED ? 1 : TK + 2], dpx[STAGED ? 1 : TK + 2];  // DPH and DPL / DPU
+  float fi[TK + 2][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
+  float fj[TK + 2][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
 
 // r, z, dinv and the FP32 sweep vector are all tile-major (TileMap): a
@@ -628,8 +629,6 @@ __global__ void __launch_bounds__(64 * W)
   constexpr bool kStaged = W > 1;
   constexpr int kQPW = TK / W;                    // planes staged per wave
   constexpr int kRound = kQPW < 16 ? kQPW : 16;   // planes per round of loads
-  // the done flag is read with the tile's data and tested before any store
-  const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
@@ -639,9 +638,11 @@ __global__ void __launch_bounds__(64 * W)
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
-  // couplings kept by the block-Jacobi cut (tile-aligned: inside a tile only
-  // the first plane can start a phi block)
-  const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
+  // The march applies every coupling: where the block-Jacobi cut (tile-
+  // aligned: inside a tile only the first plane can start a phi block) or
+  // the domain boundary drops one, the value it multiplies is zero (the
+  // incoming face, or the plane below the tile, is not loaded).
+  const bool lo_k0 = da.bl.lo_k(k0);
   const bool wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
   const double P1 = DRH * m.CP[jc];
@@ -649,57 +650,79 @@ __global__ void __launch_bounds__(64 * W)
   // this tile's block of the tile-major FP32 arrays (plane q at tb + 64 q)
   constexpr size_t kTile = (size_t)TK * 64;
   const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
-  float t_own = 0.f, t_w = 0.f;
-  if (wave == 0) {
-    if (act && lo_k0) {  // plane k0-1: the last plane of the tile below
-      const size_t q = tile_id(da, I, J, Kc - 1) * kTile + (TK - 1) * 64 + lane;
-      t_own = da.dinv_t[q] * da.yv_t[q];
-    }
-    if (act && wrap && k0 > 0 && k1 == g.c) {  // plane 0
-      const size_t q = tile_id(da, I, J, 0) * kTile + lane;
-      t_w = da.dinv_t[q] * da.yv_t[q];
+  // Every global load of the tile is issued in one batch before any of
+  // its values is used: the loads are unconditional, at clamped in-bounds
+  // addresses (a tile's arrays hold TK planes, its faces TK rows), and the
+  // values not wanted are dropped afterwards.  The incoming faces, the
+  // per-plane factors and the planes below and at the seam go first, the
+  // tile's own planes next, the done flag last; a tile then waits for
+  // memory once before its march.
+  constexpr int kFPT = (TK * 8 + 64 * W - 1) / (64 * W);  // face entries per thread
+  const bool face_i = I > 0 && da.bl.lo_i(I * TI), face_j = J > 0 && da.bl.lo_j(J * TJ);
+  float fiv[kFPT], fjv[kFPT];
+  {
+    const size_t fbi = tile_id(da, max(I - 1, 0), J, Kc) * TK * 8;
+    const size_t fbj = tile_id(da, I, max(J - 1, 0), Kc) * TK * 8;
+#pragma unroll
+    for (int e = 0; e < kFPT; e++) {
+      const int f = min((int)threadIdx.x + e * 64 * W, TK * 8 - 1);
+      fiv[e] = da.face_i[fbi + f];
+      fjv[e] = da.face_j[fbj + f];
     }
   }
-  // stage: planes wave*kQPW .. +kQPW-1, rounds of kRound
+  double dphv = 0.0, dpxv = 0.0;
+  if constexpr (!kStaged) {
+    const int kq = min(k0 + (int)threadIdx.x, g.c - 1);
+    dphv = m.DPH[kq];
+    dpxv = m.DPL[kq];
+  }
+  // plane k0-1 (the last plane of the tile below) and plane 0 (seam)
+  const size_t qo = tile_id(da, I, J, max(Kc - 1, 0)) * kTile + (TK - 1) * 64 + lane;
+  const size_t qw = tile_id(da, I, J, 0) * kTile + lane;
+  const float od = da.dinv_t[qo], oy = da.yv_t[qo];
+  const float wd = da.dinv_t[qw], wy = da.yv_t[qw];
+  // stage: planes wave*kQPW .. +kQPW-1, rounds of kRound (planes past nk
+  // are the zero padding of the tile-major arrays; their rows are scratch)
 #pragma unroll 1
   for (int q0 = wave * kQPW; q0 < (wave + 1) * kQPW; q0 += kRound) {
     float rv[kRound], dv[kRound];
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
-      const int q = q0 + u;
-      rv[u] = dv[u] = 0.f;
-      if (q < nk) {
-        dv[u] = da.dinv_t[tb + (size_t)q * 64];
-        rv[u] = (float)r[tb + (size_t)q * 64];
-      }
+      const size_t o = tb + (size_t)(q0 + u) * 64;
+      dv[u] = da.dinv_t[o];
+      rv[u] = (float)r[o];
     }
-    if (done) return;  // uniform over the block
+    // the done flag is read after the tile's loads and tested before any store
+    if (guard && st_done(guard)) return;  // uniform over the block
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
       const int q = q0 + u;
-      if (q < nk) {
-        sh.v[q][lane] = rv[u];
-        sh.d[q][lane] = dv[u];
-        if constexpr (kStaged) {
+      sh.v[q + 1][lane] = rv[u];
+      sh.d[q + 1][lane] = dv[u];
+      if constexpr (kStaged) {
+        if (q < nk) {
           const double dph = m.DPH[k0 + q];
-          sh.ck[q][lane] = (float)(-(P1 * m.DPL[k0 + q]));
-          sh.cj[q][lane] = (float)(-((DRH * dph) * CTL));
-          sh.ci[q][lane] = (float)(-((SD * dph) * CRL));
+          sh.ck[q + 1][lane] = (float)(-(P1 * m.DPL[k0 + q]));
+          sh.cj[q + 1][lane] = (float)(-((DRH * dph) * CTL));
+          sh.ci[q + 1][lane] = (float)(-((SD * dph) * CRL));
         }
       }
     }
   }
-  // incoming faces (one float per thread and round) and per-plane factors
-  const bool face_i = I > 0 && da.bl.lo_i(I * TI), face_j = J > 0 && da.bl.lo_j(J * TJ);
-  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
-    const int fq = f >> 3, fe = f & 7;
-    sh.fi[fq][fe] = face_i ? da.face_i[(tile_id(da, I - 1, J, Kc) * TK + fq) * 8 + fe] : 0.f;
-    sh.fj[fq][fe] = face_j ? da.face_j[(tile_id(da, I, J - 1, Kc) * TK + fq) * 8 + fe] : 0.f;
+  const float t_own = (wave == 0 && act && lo_k0) ? od * oy : 0.f;
+  const float t_w = (wave == 0 && act && wrap && k0 > 0 && k1 == g.c) ? wd * wy : 0.f;
+#pragma unroll
+  for (int e = 0; e < kFPT; e++) {
+    const int f = (int)threadIdx.x + e * 64 * W;
+    if (f < TK * 8) {
+      sh.fi[(f >> 3) + 1][f & 7] = face_i ? fiv[e] : 0.f;
+      sh.fj[(f >> 3) + 1][f & 7] = face_j ? fjv[e] : 0.f;
+    }
   }
   if constexpr (!kStaged) {
-    for (int q = (int)threadIdx.x; q < nk; q += 64 * W) {
-      sh.dph[q] = m.DPH[k0 + q];
-      sh.dpx[q] = m.DPL[k0 + q];
+    if ((int)threadIdx.x < TK) {
+      sh.dph[threadIdx.x + 1] = dphv;
+      sh.dpx[threadIdx.x + 1] = dpxv;
     }
   }
   __syncthreads();
@@ -720,9 +743,11 @@ __global__ void __launch_bounds__(64 * W)
       float t_k = t_init, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
+        int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qs = min(max(s0 + u - ii - jj, 0), nk - 1);
+          const int qs = min(max(s0 + u - ii - jj, -1), nk) + 1;
+          rw[u] = qs;
           rr[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
           if constexpr (kStaged) {
@@ -748,10 +773,10 @@ __global__ void __launch_bounds__(64 * W)
             if (q == nk - 1) y = y - aw * tw;
           }
           const float tk = (q > 0) ? t_k : t_init;
-          if (q > 0 || lo_k0) y = y - ak[u] * tk;
-          if (lo_j) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-          if (lo_i) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
-          sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = y;
+          y = y - ak[u] * tk;
+          y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
+          y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+          sh.v[rw[u]][lane] = y;
           const float tt = dd[u] * y;
           if constexpr (kSeam) {
             if (k0 == 0 && q == 0) tw = tt;
@@ -770,15 +795,15 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
     for (int u = 0; u < kQPW; u++) {
       const int q = wave * kQPW + u;
-      if (q < nk) da.yv_t[tb + (size_t)q * 64] = sh.v[q][lane];
+      if (q < nk) da.yv_t[tb + (size_t)q * 64] = sh.v[q + 1][lane];
     }
   }
   // outgoing faces t = dinv*y of the last column (ii = 7) and row (jj = 7)
   for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
     const int fq = f >> 3, fe = f & 7;
     const size_t tf = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
-    da.face_i[tf] = sh.d[fq][TI - 1 + TI * fe] * sh.v[fq][TI - 1 + TI * fe];
-    da.face_j[tf] = sh.d[fq][fe + TI * (TJ - 1)] * sh.v[fq][fe + TI * (TJ - 1)];
+    da.face_i[tf] = sh.d[fq + 1][TI - 1 + TI * fe] * sh.v[fq + 1][TI - 1 + TI * fe];
+    da.face_j[tf] = sh.d[fq + 1][fe + TI * (TJ - 1)] * sh.v[fq + 1][fe + TI * (TJ - 1)];
   }
 }
 
@@ -788,7 +813,6 @@ __global__ void __launch_bounds__(64 * W)
   constexpr bool kStaged = W > 1;
   constexpr int kQPW = TK / W;
   constexpr int kRound = kQPW < 16 ? kQPW : 16;
-  const int done = guard ? st_done(guard) : 0;
   __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
@@ -799,61 +823,81 @@ __global__ void __launch_bounds__(64 * W)
   const int i = I * TI + ii, j = J * TJ + jj;
   const bool act = (i < g.a) && (j < g.b);
   const int ic = act ? i : 0, jc = act ? j : 0;
-  const bool hi_i = da.bl.hi_i(i, g.a), hi_j = da.bl.hi_j(j, g.b);
+  // every coupling applied, as in the forward sweep; the upper neighbours
+  // outside the grid are padding lanes, whose dinv (hence z) is zero
   const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
   const double P1 = DRH * m.CP[jc];
   const double dpl_first = m.DPL[0];
   constexpr size_t kTile = (size_t)TK * 64;
   const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
-  float t_own = 0.f, t_w = 0.f;
-  if (wave == 0) {
-    // plane k1 (the first plane of the tile above) and plane c-1 come from
-    // the compact k faces those tiles left
-    if (act && hi_k1) t_own = da.face_k[tile_id(da, I, J, Kc + 1) * 64 + lane];
-    if (act && wrap && k1 < g.c && k0 == 0)
-      t_w = da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane];
+  // one batch of loads, as in the forward sweep: incoming faces (z of the
+  // first column / row of the upper neighbours), per-plane factors, the
+  // compact k faces (plane k1, the first plane of the tile above, and
+  // plane c-1 at the seam), the tile's planes, the done flag
+  constexpr int kFPT = (TK * 8 + 64 * W - 1) / (64 * W);  // face entries per thread
+  const bool face_i = I + 1 < da.NI && da.bl.hi_i(I * TI + TI - 1, g.a);
+  const bool face_j = J + 1 < da.NJ && da.bl.hi_j(J * TJ + TJ - 1, g.b);
+  float fiv[kFPT], fjv[kFPT];
+  {
+    const size_t fbi = tile_id(da, min(I + 1, da.NI - 1), J, Kc) * TK * 8;
+    const size_t fbj = tile_id(da, I, min(J + 1, da.NJ - 1), Kc) * TK * 8;
+#pragma unroll
+    for (int e = 0; e < kFPT; e++) {
+      const int f = min((int)threadIdx.x + e * 64 * W, TK * 8 - 1);
+      fiv[e] = da.face_i[fbi + f];
+      fjv[e] = da.face_j[fbj + f];
+    }
   }
+  double dphv = 0.0, dpxv = 0.0;
+  if constexpr (!kStaged) {
+    const int kq = min(k0 + (int)threadIdx.x, g.c - 1);
+    dphv = m.DPH[kq];
+    dpxv = m.DPU[kq];
+  }
+  const float fk = da.face_k[tile_id(da, I, J, min(Kc + 1, da.NK - 1)) * 64 + lane];
+  const float fkt = da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane];
 #pragma unroll 1
   for (int q0 = wave * kQPW; q0 < (wave + 1) * kQPW; q0 += kRound) {
     float yv8[kRound], dv[kRound];
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
-      const int q = q0 + u;
-      yv8[u] = dv[u] = 0.f;
-      if (act && q < nk) {
-        yv8[u] = da.yv_t[tb + (size_t)q * 64];
-        dv[u] = da.dinv_t[tb + (size_t)q * 64];
-      }
+      const size_t o = tb + (size_t)(q0 + u) * 64;
+      yv8[u] = da.yv_t[o];
+      dv[u] = da.dinv_t[o];
     }
-    if (done) return;  // uniform over the block
+    if (guard && st_done(guard)) return;  // uniform over the block
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
       const int q = q0 + u;
-      if (q < nk) {
-        sh.v[q][lane] = yv8[u];
-        sh.d[q][lane] = dv[u];
-        if constexpr (kStaged) {
+      // y outside the grid or past plane nk-1 is never written: zero it
+      const bool in = act && q < nk;
+      sh.v[q + 1][lane] = in ? yv8[u] : 0.f;
+      sh.d[q + 1][lane] = in ? dv[u] : 0.f;
+      if constexpr (kStaged) {
+        if (q < nk) {
           const double dph = m.DPH[k0 + q];
-          sh.ck[q][lane] = (float)(-(P1 * m.DPU[k0 + q]));
-          sh.cj[q][lane] = (float)(-((DRH * dph) * CTH));
-          sh.ci[q][lane] = (float)(-((SD * dph) * CRH));
+          sh.ck[q + 1][lane] = (float)(-(P1 * m.DPU[k0 + q]));
+          sh.cj[q + 1][lane] = (float)(-((DRH * dph) * CTH));
+          sh.ci[q + 1][lane] = (float)(-((SD * dph) * CRH));
         }
       }
     }
   }
-  // incoming faces (z of the first column / row of the upper neighbours)
-  const bool face_i = I + 1 < da.NI && da.bl.hi_i(I * TI + TI - 1, g.a);
-  const bool face_j = J + 1 < da.NJ && da.bl.hi_j(J * TJ + TJ - 1, g.b);
-  for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
-    const int fq = f >> 3, fe = f & 7;
-    sh.fi[fq][fe] = face_i ? da.face_i[(tile_id(da, I + 1, J, Kc) * TK + fq) * 8 + fe] : 0.f;
-    sh.fj[fq][fe] = face_j ? da.face_j[(tile_id(da, I, J + 1, Kc) * TK + fq) * 8 + fe] : 0.f;
+  const float t_own = (wave == 0 && act && hi_k1) ? fk : 0.f;
+  const float t_w = (wave == 0 && act && wrap && k1 < g.c && k0 == 0) ? fkt : 0.f;
+#pragma unroll
+  for (int e = 0; e < kFPT; e++) {
+    const int f = (int)threadIdx.x + e * 64 * W;
+    if (f < TK * 8) {
+      sh.fi[(f >> 3) + 1][f & 7] = face_i ? fiv[e] : 0.f;
+      sh.fj[(f >> 3) + 1][f & 7] = face_j ? fjv[e] : 0.f;
+    }
   }
   if constexpr (!kStaged) {
-    for (int q = (int)threadIdx.x; q < nk; q += 64 * W) {
-      sh.dph[q] = m.DPH[k0 + q];
-      sh.dpx[q] = m.DPU[k0 + q];
+    if ((int)threadIdx.x < TK) {
+      sh.dph[threadIdx.x + 1] = dphv;
+      sh.dpx[threadIdx.x + 1] = dpxv;
     }
   }
   __syncthreads();
@@ -870,9 +914,11 @@ __global__ void __launch_bounds__(64 * W)
       float t_k = t_init, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
+        int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), 0), nk - 1);
+          const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), -1), nk) + 1;
+          rw[u] = qs;
           yy[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
           if constexpr (kStaged) {
@@ -894,15 +940,15 @@ __global__ void __launch_bounds__(64 * W)
           float from_i, from_j;
           lanes_from_upper(t_k, from_i, from_j);
           float sv = yy[u];
-          if (hi_i) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-          if (hi_j) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+          sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
+          sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
           const float tk = (q < nk - 1) ? t_k : t_init;
-          if (q < nk - 1 || hi_k1) sv = sv - bk[u] * tk;
+          sv = sv - bk[u] * tk;
           if constexpr (kSeam) {
             if (q == 0) sv = sv - bw * tw;
           }
           const float zz = sv * dd[u];
-          sh.v[((unsigned)q < (unsigned)nk) ? q : TK][lane] = zz;
+          sh.v[rw[u]][lane] = zz;
           if constexpr (kSeam) {
             if (k1 == g.c && q == nk - 1) tw = zz;
           }
@@ -920,19 +966,19 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
     for (int u = 0; u < kQPW; u++) {
       const int q = wave * kQPW + u;
-      if (q < nk) z[tb + (size_t)q * 64] = (T)sh.v[q][lane];
+      if (q < nk) z[tb + (size_t)q * 64] = (T)sh.v[q + 1][lane];
     }
     if (wave == 0) {  // k faces: the first plane, and plane c-1 for the seam
-      da.face_k[tile_id(da, I, J, Kc) * 64 + lane] = sh.v[0][lane];
-      if (k1 == g.c) da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane] = sh.v[nk - 1][lane];
+      da.face_k[tile_id(da, I, J, Kc) * 64 + lane] = sh.v[1][lane];
+      if (k1 == g.c) da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane] = sh.v[nk][lane];
     }
   }
   // outgoing faces: z of the first column (ii = 0) and row (jj = 0)
   for (int f = (int)threadIdx.x; f < nk * 8; f += 64 * W) {
     const int fq = f >> 3, fe = f & 7;
     const size_t tf = (tile_id(da, I, J, Kc) * TK + fq) * 8 + fe;
-    da.face_i[tf] = sh.v[fq][TI * fe];
-    da.face_j[tf] = sh.v[fq][fe];
+    da.face_i[tf] = sh.v[fq + 1][TI * fe];
+    da.face_j[tf] = sh.v[fq + 1][fe];
   }
 }
 
