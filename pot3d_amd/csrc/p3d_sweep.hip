@@ -27,6 +27,9 @@ namespace {
 
 constexpr int TI = 8, TJ = 8;
 constexpr int SKEW = TI + TJ - 2;  // steps between the first and last lane of a tile
+// planes a wave loads per round when it stages a tile (rounds of 32 for
+// 32-plane tiles measured no faster)
+constexpr int kTileRound = 16;
 constexpr int U = 16;              // steps per prefetch block
 constexpr unsigned kSpinMax = 1u << 20;
 }  // namespace
@@ -599,9 +602,9 @@ struct TileLds {
   float v[TK + 2][64];  // r (forward) or y (backward), overwritten by the result
   float d[TK + 2][64];  // dinv
   // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
-  float ck[STAGED ? TK + 2 : 1][64];
-  float cj[STAGED ? TK + 2 : 1][64];
-  float ci[STAGED ? TK + 2 : 1][64];
+  float ck[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
+  float cj[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
+  float ci[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
   double dph[STAGED ? 1 : TK + 2], dpx[STAGED ? 1 : TK + 2];  // DPH and DPL / DPU
   float fi[TK + 2][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
   float fj[TK + 2][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
@@ -628,7 +631,7 @@ __global__ void __launch_bounds__(64 * W)
                    const CgState* guard) {
   constexpr bool kStaged = W > 1;
   constexpr int kQPW = TK / W;                    // planes staged per wave
-  constexpr int kRound = kQPW < 16 ? kQPW : 16;   // planes per round of loads
+  constexpr int kRound = kQPW < kTileRound ? kQPW : kTileRound;  // planes per round of loads
   __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
@@ -812,7 +815,7 @@ __global__ void __launch_bounds__(64 * W)
     k_tile_bwd_lds(Metric m, Grid3 g, DiagArgs da, T* __restrict__ z, const CgState* guard) {
   constexpr bool kStaged = W > 1;
   constexpr int kQPW = TK / W;
-  constexpr int kRound = kQPW < 16 ? kQPW : 16;
+  constexpr int kRound = kQPW < kTileRound ? kQPW : kTileRound;
   __shared__ TileLds<TK, kStaged> sh;
   int I, J, Kc;
   diag_tile(da, blockIdx.x, I, J, Kc);
