@@ -485,6 +485,11 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
                         s->stream));
   // the recurrence's solution updates go straight to the FP64 x
   Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32, s->x, s->out + 4};
+  // the residual update runs over every entry of the tile-major r and ap:
+  // entries outside the grid must be zero in both (other users of the
+  // buffers may have written them)
+  HIPCHK(hipMemsetAsync(s->ap32, 0, s->nvec * sizeof(float), s->stream));
+  HIPCHK(hipMemsetAsync(s->r32, 0, s->nvec * sizeof(float), s->stream));
   launch_cast_scale<float, double>(s->g, v.r, s->ap, s->out + 5, s->out + 4, 0, s->stream,
                                    s->tm);
   // the inner relres = ||r32|| scale / ||b|| estimates the true one
@@ -1197,9 +1202,9 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
         launch_stencil<double>(s->m, s->g, s->L, s->p, s->ap, nullptr, 0, s->pole_part, nullptr,
                                s->red, 0, EPI_NONE, nullptr, false, s->stream);
         break;
-      case 1:
+      case 1:  // as in the FP32 recurrence: ap placed by the tile map
         launch_stencil<float>(s->m, s->g, s->L, s->p32, s->ap32, nullptr, 0, s->pole_part,
-                              nullptr, s->red, 0, EPI_NONE, nullptr, false, s->stream);
+                              nullptr, s->red, 0, EPI_NONE, nullptr, false, s->stream, s->tm);
         break;
       case 2:
         prec_and_dot<double>(s, s->r, s->z, EPI_NONE, nullptr, false);
