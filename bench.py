@@ -139,6 +139,7 @@ VALUE_KERNELS = [
 def set_fast_ilu(s, args):
     """The block-Jacobi ILU0 of the fast paths (the reference's per-rank
     ILU0 on a grid decomposed into that many blocks)."""
+    from pot3d_amd import _lib
     blocks = tuple(int(v) for v in args.blocks.split(","))
     for key, v in zip((_lib.OPT_ILU_BLOCKS_R, _lib.OPT_ILU_BLOCKS_T, _lib.OPT_ILU_BLOCKS_P),
                       blocks):
