@@ -133,6 +133,11 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
 VALUE_KERNELS = [
     ("ilu_apply_fp32_blockjacobi (k_tile_fwd_lds+k_tile_bwd_lds<float>)", 7, 24, 0),
     ("stencil_fp32 (k_stencil_march<float>)", 1, 4, 4),
+    # p, z (tile-major, gathered), FP64 solution: 4+4+8 read, 4+8 written
+    ("p_update_fp32_fp64x (k_p_update<float>)", 8, 28, 0),
+    # r, ap read, r written (tile-major)
+    ("r_update_fp32 (k_r_update)", 9, 12, 0),
+    ("rz_dot_fp32 (k_dot2<float>, tile order)", 10, 8, 0),
 ]
 
 

@@ -481,10 +481,11 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n1 = min(n_all, n0 + (uint32_t)CH);
   const float alpha = (float)st_load(&st->alpha);
   double s1 = 0.0;
-  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
-    float rv[kUnroll], av[kUnroll];
+  constexpr int kU = 8;  // points per thread in flight
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kU * kBlock) {
+    float rv[kU], av[kU];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kU; u++) {
       const uint32_t n = nb + u * kBlock;
       rv[u] = av[u] = 0.f;
       if (n < n1) {
@@ -493,7 +494,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kU; u++) {
       const uint32_t n = nb + u * kBlock;
       if (n < n1) {
         const float rn = rv[u] - alpha * av[u];
@@ -533,25 +534,30 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   const T beta = (T)st_load(&st->beta);
   const double alpha_x = do_x ? st_load(&st->alpha) * *xscale : 0.0;
-  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
-    T pv[kUnroll], zv[kUnroll];
-    double xv[kUnroll];
+  // 16 points per thread in flight, and the p and solution streams read
+  // (and the solution written) non-temporally: neither is reused before it
+  // has left the caches, and keeping them out leaves room for the gathered
+  // z (measured: 1.60 -> 1.27 ms at 214.7 M points)
+  constexpr int kU = 16;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kU * kBlock) {
+    T pv[kU], zv[kU];
+    double xv[kU];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kU; u++) {
       const uint32_t n = nb + u * kBlock;
       pv[u] = zv[u] = (T)0;
       xv[u] = 0.0;
       if (n < n1) {
-        pv[u] = p[n];
-        if (do_x) xv[u] = xs[n];
+        pv[u] = __builtin_nontemporal_load(p + n);
+        if (do_x) xv[u] = __builtin_nontemporal_load(xs + n);
         if (!done) zv[u] = (prec == PREC_ILU) ? z[tmap(tm, g, n)] : r[n];
       }
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kU; u++) {
       const uint32_t n = nb + u * kBlock;
       if (n < n1) {
-        if (do_x) xs[n] = xv[u] + alpha_x * (double)pv[u];
+        if (do_x) __builtin_nontemporal_store(xv[u] + alpha_x * (double)pv[u], xs + n);
         if (!done) {
           const T zn = (prec == PREC_ILU) ? zv[u] : apply_point_prec<T>(m, g, prec, n, zv[u]);
           p[n] = beta * pv[u] + zn;
@@ -593,16 +599,18 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
   const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
   double s0 = 0.0, s1 = 0.0;
-  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kUnroll * kBlock) {
-    T xv[kUnroll], yv[kUnroll], uv[kUnroll], vv[kUnroll];
+  // 8 points per thread in flight; x and y read non-temporally (streams)
+  constexpr int kU = 8;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kU * kBlock) {
+    T xv[kU], yv[kU], uv[kU], vv[kU];
 #pragma unroll
-    for (int q = 0; q < kUnroll; q++) {
+    for (int q = 0; q < kU; q++) {
       const uint32_t n = nb + q * kBlock;
       xv[q] = yv[q] = uv[q] = vv[q] = (T)0;
       if (n < n1) {
         const uint32_t t = tmap(tm, g, n);
-        xv[q] = x[t];
-        yv[q] = y[t];
+        xv[q] = __builtin_nontemporal_load(x + t);
+        yv[q] = __builtin_nontemporal_load(y + t);
         if (u) {
           uv[q] = u[t];
           vv[q] = v[t];
@@ -610,7 +618,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int q = 0; q < kUnroll; q++) {
+    for (int q = 0; q < kU; q++) {
       const uint32_t n = nb + q * kBlock;
       if (n < n1) {
         s0 += (double)xv[q] * (double)yv[q];

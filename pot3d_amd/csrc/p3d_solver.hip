@@ -1153,7 +1153,9 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   if (!s || !avg_ms || reps <= 0) return fail(P3D_EINVAL, "bad argument");
   if (which == 2 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
   if (which == 7 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
-  if (which == 1 || which == 3 || which == 7) {
+  if ((which == 8 || which == 9 || which == 10) && !s->tm.on)
+    return fail(P3D_ESTATE, "no tile-major placement (single-wave ILU tile sweeps)");
+  if (which == 1 || which == 3 || which >= 7) {
     int rc = ensure_mixed(s);
     if (rc) return rc;
   }
@@ -1167,7 +1169,11 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   init.bb = 1.0;
   init.epscg = 0.0;
   init.ncgmax = 1 << 30;
+  init.ncg = 1;
   *s->st_host = init;
+  s->out_host[4] = 1.0;
+  HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
   HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice, s->stream));
   Vecs<double> v{s->x, s->r, s->p, s->ap, s->z};
   if (which == 3) {
@@ -1223,6 +1229,20 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
       case 7:  // FP32 ILU apply (the mixed-precision inner preconditioner)
         if (s->ifprec == PREC_ILU) prec_and_dot<float>(s, s->r32, s->z32, EPI_NONE, nullptr, false);
         break;
+      case 8:  // p update of the FP32 recurrence with the FP64 solution update
+        // (x_ncg cleared so that every launch applies it; alpha = 0 leaves x)
+        (void)hipMemsetAsync(&s->st->x_ncg, 0, sizeof(int), s->stream);  // checked below
+        launch_p_update<float>(s->m, s->g, s->L, s->ifprec, s->p32, s->r32, s->z32, s->st,
+                               s->stream, s->tm, s->x, s->out + 4,
+                               s->red.counter + 5 * kRedCounters);
+        break;
+      case 9:  // residual update of the FP32 recurrence (tile-major r, ap)
+        launch_r_update(s->gt.N, s->Lt, s->r32, s->ap32, s->st, s->red, 2, s->stream);
+        break;
+      case 10:  // r.z of the FP32 recurrence, contiguous in tile order
+        launch_dot2<float>(s->gt, s->Lt, s->r32, s->z32, nullptr, nullptr, s->st, s->red, 1,
+                           EPI_NONE, nullptr, false, s->stream);
+        break;
       default:
         break;
     }
@@ -1238,7 +1258,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   *avg_ms = (double)ms / reps;
-  if (which > 7 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
+  if (which > 10 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
   return P3D_OK;
 }
 
