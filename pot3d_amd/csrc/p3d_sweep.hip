@@ -101,7 +101,11 @@ __device__ __forceinline__ void lanes_from_lower(float t, float& from_i, float& 
   const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
   const auto c = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);  // {r1 x4}, {r3 x4}
   const int row = lane >> 4;
-  const int x = (lane & 8) ? sh8 : (row == 2 ? (int)c[0] : (int)a[0]);
+  // the same choice as (lane & 8) ? sh8 : (row == 2 ? c : a), with the
+  // select that needs only the first swap done first: one level less on
+  // the march's dependency chain
+  const int x1 = (lane & 8) ? sh8 : (int)a[0];
+  const int x = (row == 2 && !(lane & 8)) ? (int)c[0] : x1;
   from_j = __int_as_float(x);
 }
 
@@ -114,7 +118,9 @@ __device__ __forceinline__ void lanes_from_upper(float t, float& from_i, float& 
   const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
   const auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);  // {r0 x4}, {r2 x4}
   const int row = lane >> 4;
-  const int x = (lane & 8) ? (row == 1 ? (int)c[1] : (int)a[1]) : sh8;
+  // as in lanes_from_lower: the select on the first swap's result first
+  const int x1 = (lane & 8) ? (int)a[1] : sh8;
+  const int x = (row == 1 && (lane & 8)) ? (int)c[1] : x1;
   from_j = __int_as_float(x);
 }
 
@@ -596,18 +602,19 @@ __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int 
 // tiles resident per CU).
 template <int TK, bool STAGED>
 struct TileLds {
-  // per plane and lane: plane q in row q+1; rows 0 and nk+1 are scratch,
-  // read and written by lanes whose step falls outside the tile's planes
-  // (one clamped row index serves every access of a step)
-  float v[TK + 2][64];  // r (forward) or y (backward), overwritten by the result
-  float d[TK + 2][64];  // dinv
+  // per plane and lane: plane q in row q+1; row 0 is scratch, read and
+  // written by lanes whose step falls outside the tile's planes (one row
+  // index serves every access of a step; 16-plane tiles stay under 10 KB,
+  // so 16 fit in a CU's LDS)
+  float v[TK + 1][64];  // r (forward) or y (backward), overwritten by the result
+  float d[TK + 1][64];  // dinv
   // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
-  float ck[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
-  float cj[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
-  float ci[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
-  double dph[STAGED ? 1 : TK + 2], dpx[STAGED ? 1 : TK + 2];  // DPH and DPL / DPU
-  float fi[TK + 2][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
-  float fj[TK + 2][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
+  float ck[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
+  float cj[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
+  float ci[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
+  double dph[STAGED ? 1 : TK + 1], dpx[STAGED ? 1 : TK + 1];  // DPH and DPL / DPU
+  float fi[TK + 1][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
+  float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
 
 // r, z, dinv and the FP32 sweep vector are all tile-major (TileMap): a
@@ -749,7 +756,8 @@ __global__ void __launch_bounds__(64 * W)
         int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qs = min(max(s0 + u - ii - jj, -1), nk) + 1;
+          const int qf = s0 + u - ii - jj;
+          const int qs = (unsigned)qf < (unsigned)nk ? qf + 1 : 0;
           rw[u] = qs;
           rr[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
@@ -920,7 +928,8 @@ __global__ void __launch_bounds__(64 * W)
         int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), -1), nk) + 1;
+          const int qf = (nk - 1) - (s0 + u - rii - rjj);
+          const int qs = (unsigned)qf < (unsigned)nk ? qf + 1 : 0;
           rw[u] = qs;
           yy[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
