@@ -205,9 +205,14 @@ int p3d_get_stats(p3d_solver* s, p3d_stats* st);
 int p3d_history(p3d_solver* s, double* hist, int n);
 
 /* Kernel timing with HIP events on the solver's stream: `which` =
- * 0 FP64 operator, 1 FP32 operator, 2 ILU apply (FP64 in/out),
- * 3 one FP64 PCG iteration (replayed through the solve's captured graph), 4 FP64 CG vector update, 5 FP64 p update,
- * 6 plain copy of N doubles (bandwidth reference), 7 ILU apply in FP32.
+ * 0 FP64 operator, 1 FP32 operator (output placed as in the FP32
+ * recurrence), 2 ILU apply (FP64 in/out), 3 one FP64 PCG iteration
+ * (replayed through the solve's captured graph), 4 FP64 CG vector update,
+ * 5 FP64 p update, 6 plain copy of N doubles (bandwidth reference), 7 ILU
+ * apply in FP32; with the single-wave ILU tile sweeps (tile-major vectors,
+ * else P3D_ESTATE) the FP32 recurrence's 8 p update with the FP64 solution
+ * update, 9 residual update, 10 r.z.  The work vectors' contents are
+ * left undefined (a later solve sets them up).
  * Average milliseconds per launch over reps launches. */
 int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms);
 

@@ -237,6 +237,13 @@ __global__ void __launch_bounds__(kBlock)
 // reduced per block with the same last-block epilogue.
 constexpr int kSegPlanes = 16;
 
+// Column blocks of the operator march: kBlock consecutive columns, or, with
+// a tile-major output, kBlock/64 patches of 8x8 columns.
+__host__ __device__ inline int stencil_col_blocks(const Grid3& g, const TileMap& tm) {
+  if (tm.on) return (tm.TNI * tm.TNJ + kBlock / 64 - 1) / (kBlock / 64);
+  return (int)((g.ab + kBlock - 1) / kBlock);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_stencil_march(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
@@ -246,15 +253,31 @@ __global__ void __launch_bounds__(kBlock)
   if (guard && st_done(st)) return;
   const int G = gridDim.x;
   const int Lb = logical_block(G);
-  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
+  const int ncol_blocks = stencil_col_blocks(g, tm);
   const int cb = Lb % ncol_blocks, seg = Lb / ncol_blocks;
   const int kper = (g.c + nseg - 1) / nseg;
   const int k0 = seg * kper, k1 = min(g.c, k0 + kper);
-  const uint32_t mcol = (uint32_t)cb * kBlock + threadIdx.x;
+  // the columns of a wave: 64 consecutive mcol = i + a j, or, with the
+  // tile-major output, one 8x8 (i, j) patch, so that a wave writes whole
+  // 256-byte tile planes
+  int i, j;
+  bool col_ok;
+  if (tm.on) {
+    const int t = cb * (kBlock / 64) + (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    i = (t % tm.TNI) * 8 + (lane & 7);
+    j = (t / tm.TNI) * 8 + (lane >> 3);
+    col_ok = t < tm.TNI * tm.TNJ && i < g.a && j < g.b;
+  } else {
+    const uint32_t mc = (uint32_t)cb * kBlock + threadIdx.x;
+    const uint32_t jq = fdiv(mc, g.da);
+    i = (int)(mc - jq * (uint32_t)g.a);
+    j = (int)jq;
+    col_ok = mc < g.ab;
+  }
   double acc = 0.0;
-  if (mcol < g.ab && k0 < k1) {
-    const uint32_t jj = fdiv(mcol, g.da);
-    const int i = (int)(mcol - jj * (uint32_t)g.a), j = (int)jj;
+  if (col_ok && k0 < k1) {
+    const uint32_t mcol = (uint32_t)i + (uint32_t)g.a * (uint32_t)j;
     const double DRH = m.DRH[i], CRL = m.CRL[i], CRH = m.CRH[i];
     const double SD = m.SD[j], CTL = m.CTL[j], CTH = m.CTH[j];
     const double P1 = DRH * m.CP[j];
@@ -345,9 +368,9 @@ __global__ void __launch_bounds__(kBlock)
   block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
 }
 
-static int stencil_segments(const Grid3& g) {
+static int stencil_segments(const Grid3& g, const TileMap& tm) {
   // enough blocks to fill the chip several times, >= kSegPlanes planes each
-  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
+  const int ncol_blocks = stencil_col_blocks(g, tm);
   int nseg = (8 * 256 * 4 + ncol_blocks - 1) / ncol_blocks;
   const int maxseg = (g.c + kSegPlanes - 1) / kSegPlanes;
   if (nseg > maxseg) nseg = maxseg;
@@ -355,9 +378,8 @@ static int stencil_segments(const Grid3& g) {
   return nseg;
 }
 
-int stencil_blocks(const Grid3& g) {
-  const int ncol_blocks = (int)((g.ab + kBlock - 1) / kBlock);
-  const int G = ncol_blocks * stencil_segments(g);
+int stencil_blocks(const Grid3& g, const TileMap& tm) {
+  const int G = stencil_col_blocks(g, tm) * stencil_segments(g, tm);
   return (G + 7) / 8 * 8;
 }
 
@@ -368,8 +390,8 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     const TileMap& tm) {
   (void)L;
   const int want = (epi != EPI_NONE);
-  const int nseg = stencil_segments(g);
-  const int G = stencil_blocks(g);
+  const int nseg = stencil_segments(g, tm);
+  const int G = stencil_blocks(g, tm);
   k_stencil_march<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
                                            red.partial, red.counter + (size_t)slot * kRedCounters,
                                            epi, out, guard ? 1 : 0, want, tm);

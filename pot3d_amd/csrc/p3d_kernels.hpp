@@ -107,8 +107,9 @@ Launch plan_stream(uint32_t N, int max_blocks);
 template <typename T>
 void launch_pole_sums(const Metric& m, const Grid3& g, const T* x, double* part, hipStream_t s);
 
-// Workgroups of the operator launch (its reductions need that many slots).
-int stencil_blocks(const Grid3& g);
+// Workgroups of the operator launch (its reductions need that many slots);
+// tm: the placement of its output (natural by default).
+int stencil_blocks(const Grid3& g, const TileMap& tm = TileMap{});
 
 // y = A x (homogeneous boundary values; mode 0) or y = b - A x (mode 1).
 // Reduces s0 = x.y (mode 0) or s0 = y.y (mode 1) and runs epilogue `epi`.

@@ -787,7 +787,15 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   if (hipHostMalloc((void**)&s->out_host, 8 * sizeof(double), 0) != hipSuccess)
     return bail(fail(P3D_ENOMEM, "hipHostMalloc failed"));
   if ((rc = dalloc((void**)&s->out, 8 * sizeof(double)))) return bail(rc);
-  s->red.max_blocks = s->L.G > stencil_blocks(s->g) ? s->L.G : stencil_blocks(s->g);
+  {
+    // the operator's launch with a natural or a tile-major (8x8 columns) output
+    TileMap tsz;
+    tsz.on = 1;
+    tsz.TNI = (a + 7) / 8;
+    tsz.TNJ = (b + 7) / 8;
+    const int sb = std::max(stencil_blocks(s->g), stencil_blocks(s->g, tsz));
+    s->red.max_blocks = s->L.G > sb ? s->L.G : sb;
+  }
   if (s->red.max_blocks > kRedGroup * (kRedCounters - 1))
     return bail(fail(P3D_EINVAL, "grid too large for the reduction counters"));
   if ((rc = dalloc((void**)&s->red.partial,
