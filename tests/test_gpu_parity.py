@@ -308,7 +308,7 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
         # the kernel timings write the FP32 work vectors (the FP32 operator's
         # output in natural order included): the solve must not depend on
         # what they leave there
-        for kid in (1, 7):
+        for kid in (1, 7, 8, 9, 10):
             s.time_kernel(kid, 1)
         x = np.zeros(s.N)
         assert s.solve(x, None, 1e-9, ncgmax=5000)[0] == 0
