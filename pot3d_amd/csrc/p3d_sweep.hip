@@ -651,7 +651,7 @@ __global__ void __launch_bounds__(64 * W)
   // The march applies every coupling: where the block-Jacobi cut (tile-
   // aligned: inside a tile only the first plane can start a phi block) or
   // the domain boundary drops one, the value it multiplies is zero (the
-  // incoming face, or the plane below the tile, is not loaded).
+  // incoming face, or the plane below the tile, is zeroed after loading).
   const bool lo_k0 = da.bl.lo_k(k0);
   const bool wrap = da.bl.wrap != 0;
   const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
@@ -692,7 +692,8 @@ __global__ void __launch_bounds__(64 * W)
   const float od = da.dinv_t[qo], oy = da.yv_t[qo];
   const float wd = da.dinv_t[qw], wy = da.yv_t[qw];
   // stage: planes wave*kQPW .. +kQPW-1, rounds of kRound (planes past nk
-  // are the zero padding of the tile-major arrays; their rows are scratch)
+  // are the zero padding of the tile-major arrays; the march does not read
+  // their rows)
 #pragma unroll 1
   for (int q0 = wave * kQPW; q0 < (wave + 1) * kQPW; q0 += kRound) {
     float rv[kRound], dv[kRound];
