@@ -320,6 +320,22 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
         assert _rel(x, x64) < 1e-8
         s.fill_phi()
         assert np.allclose(s.magnetic_energy(), w64, rtol=1e-7, atol=0)
+        # the benchmark's configuration: block-Jacobi cut, 16-plane tiles
+        # staged by the marching wave (ragged edges: 39 x 79 x 158 points)
+        for key, v in zip((_lib.OPT_ILU_BLOCKS_R, _lib.OPT_ILU_BLOCKS_T, _lib.OPT_ILU_BLOCKS_P),
+                          (2, 4, 2)):
+            s.set_option(key, v)
+        s.set_option(_lib.OPT_TILE_K, 16)
+        s.set_option(_lib.OPT_TILE_STAGE, 1)
+        x = np.zeros(s.N)
+        assert s.solve(x, None, 1e-9, ncgmax=5000)[0] == 0
+        st = s.stats()
+        say(f"  block-Jacobi 2x4x2, 16-plane tiles: ncg={st['ncg']} "
+            f"replacements={st['n_outer']} relres={st['true_relres']:.3e}")
+        assert st["true_relres"] <= 1e-9 and st["ncg"] <= 1.3 * n64 + 10
+        assert _rel(x, x64) < 1e-8
+        s.fill_phi()
+        assert np.allclose(s.magnetic_energy(), w64, rtol=1e-7, atol=0)
 
 
 def test_failure_paths(oracle_mod, solver_mod):
