@@ -84,44 +84,27 @@ __device__ __forceinline__ unsigned take_ticket(unsigned* ctr) {
 // Neighbour exchange inside one wave holding an 8x8 block of columns, lane
 // l = ii + 8 jj.  lanes_from_lower gives lane l the values of lanes l-1 and
 // l-8, lanes_from_upper those of l+1 and l+8 (meaningful where those lanes
-// are in the block).  DPP row shifts move data inside a 16-lane row; the
-// j step that crosses rows takes a rotate by 8 and the permlane16/32 swaps
-// (rows 0<->1 and 2<->3, then halves of the wave).  Everything is VALU: the
-// march's dependency chain makes no LDS round trip.
+// are in the block; the edge lanes take a face value instead).  The i step
+// is a DPP row shift (VALU).  The j step crosses 16-lane rows: one
+// ds_bpermute through the LDS crossbar, no VALU work.  A VALU-only version
+// (DPP rotate by 8, then the gfx950 permlane16/32 swaps and two selects)
+// measured 1-2% slower in the tile sweeps, whose SIMDs are kept busy by the
+// resident waves' VALU work.
 #define P3D_DPP(v, ctrl) __builtin_amdgcn_mov_dpp((v), (ctrl), 0xf, 0xf, true)
-constexpr int kDppRowShl1 = 0x101, kDppRowShl8 = 0x108, kDppRowShr1 = 0x111,
-              kDppRowShr8 = 0x118, kDppRowRor8 = 0x128;
+constexpr int kDppRowShl1 = 0x101, kDppRowShr1 = 0x111;
 
 __device__ __forceinline__ void lanes_from_lower(float t, float& from_i, float& from_j) {
   const int v = __float_as_int(t);
   const int lane = threadIdx.x & 63;
   from_i = __int_as_float(P3D_DPP(v, kDppRowShr1));
-  const int sh8 = P3D_DPP(v, kDppRowShr8);      // lanes 8..15 of a row: l-8, same row
-  const int w = P3D_DPP(v, kDppRowRor8);        // lanes 0..7 of row r: lanes 8..15 of row r
-  const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
-  const auto c = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);  // {r1 x4}, {r3 x4}
-  const int row = lane >> 4;
-  // the same choice as (lane & 8) ? sh8 : (row == 2 ? c : a), with the
-  // select that needs only the first swap done first: one level less on
-  // the march's dependency chain
-  const int x1 = (lane & 8) ? sh8 : (int)a[0];
-  const int x = (row == 2 && !(lane & 8)) ? (int)c[0] : x1;
-  from_j = __int_as_float(x);
+  from_j = __int_as_float(__builtin_amdgcn_ds_bpermute(((lane - 8) & 63) << 2, v));
 }
 
 __device__ __forceinline__ void lanes_from_upper(float t, float& from_i, float& from_j) {
   const int v = __float_as_int(t);
   const int lane = threadIdx.x & 63;
   from_i = __int_as_float(P3D_DPP(v, kDppRowShl1));
-  const int sh8 = P3D_DPP(v, kDppRowShl8);      // lanes 0..7 of a row: l+8, same row
-  const int w = P3D_DPP(v, kDppRowRor8);        // lanes 8..15 of row r: lanes 0..7 of row r
-  const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);  // {r0 r0 r2 r2}, {r1 r1 r3 r3}
-  const auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);  // {r0 x4}, {r2 x4}
-  const int row = lane >> 4;
-  // as in lanes_from_lower: the select on the first swap's result first
-  const int x1 = (lane & 8) ? (int)a[1] : sh8;
-  const int x = (row == 1 && (lane & 8)) ? (int)c[1] : x1;
-  from_j = __int_as_float(x);
+  from_j = __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 8) & 63) << 2, v));
 }
 
 // Last wave out re-arms the ticket and progress counters for the next launch.
