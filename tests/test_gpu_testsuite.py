@@ -1,5 +1,5 @@
 """End-to-end on the GPU: the reference test suite's three decks
-(testsuite/*/input), solved by libpot3d_hip.so and judged by the
+(testsuite/*/input), solved by libpot3d_hip.so and checked by the
 reference's own rule -- energies equal to 7 decimal places
 (bin/pot3d_validation.sh) -- plus the recorded CG iteration counts
 (ifprec=1).  The ILU0 solve (-pc=2 in run_test_suite.sh) must give the
