@@ -39,34 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-
-class Hip:
-    """Minimal HIP runtime access for device buffers (no torch needed)."""
-
-    def __init__(self):
-        self.L = C.CDLL("libamdhip64.so")
-        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.L.hipFree.argtypes = [C.c_void_p]
-        self.L.hipDeviceSynchronize.argtypes = []
-
-    def malloc(self, n):
-        p = C.c_void_p()
-        if self.L.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
-            raise MemoryError("hipMalloc failed")
-        return p.value
-
-    def h2d(self, d, h: np.ndarray):
-        assert self.L.hipMemcpy(C.c_void_p(d), h.ctypes.data, h.nbytes, 1) == 0
-
-    def d2h(self, h: np.ndarray, d):
-        assert self.L.hipMemcpy(h.ctypes.data, C.c_void_p(d), h.nbytes, 2) == 0
-
-    def free(self, d):
-        self.L.hipFree(C.c_void_p(d))
-
-    def sync(self):
-        self.L.hipDeviceSynchronize()
+from pot3d_amd._hip import Hip  # noqa: E402
 
 
 def cpu_baseline(workload, gpu_iters: int, sample_iters: int, threads: int):

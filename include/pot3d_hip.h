@@ -76,7 +76,9 @@ enum {
   P3D_ENOMEM = 2,     /* device allocation failed */
   P3D_EHIP = 3,       /* HIP runtime error */
   P3D_ENOCONV = 4,    /* ncgmax reached (the reference's ierr=1) */
-  P3D_ENONFINITE = 5, /* NaN/Inf in the residual (e.g. non-finite br0) */
+  P3D_ENONFINITE = 5, /* NaN/Inf in the residual (e.g. non-finite br0), or
+                         a zero / non-finite ILU0 pivot (p3d_create,
+                         block options) */
   P3D_ESTATE = 6      /* call out of order (e.g. solve before set_br0) */
 };
 
@@ -100,9 +102,7 @@ enum {
   P3D_OPT_GRAPH = 5,      /* 1: replay iterations as a hipGraph (default) */
   P3D_OPT_SWEEP = 6,      /* ILU triangular-solve schedule: 0 tile diagonals
                              (8x8x32 tiles, one launch per anti-diagonal;
-                             default), 1 one launch per i+j+k hyperplane,
-                             2 one persistent launch per sweep (single
-                             applications only; rejected by the solve) */
+                             default), 1 one launch per i+j+k hyperplane */
   P3D_OPT_VERBOSE = 7,    /* 1: print the reference's residual lines */
   P3D_OPT_GUESS_CHECK = 8, /* before solving, replace an initial guess that
                              is worse than zero by zero (for guesses from

@@ -71,6 +71,8 @@ def lib():
             "orc_energy": (None, [G, _dp, _dp, _dp, _dp]),
             "orc_phi_energy": (None, [G, _dp, _dp]),
             "orc_dot": (C.c_double, [_dp, _dp, C.c_int64]),
+            "orc_cgsolve_ranks": (C.c_int, [G, _dp, _ip, _ip, _ip, _ip, C.c_int, C.c_double,
+                                            C.c_int, _dp, _dp, _ip, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -78,6 +80,40 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def rank_topology(nr: int, nt: int, np_: int, nproc: int):
+    """The processor topology (nproc_r, nproc_t, nproc_p) the reference picks
+    for nproc MPI ranks when none is prescribed (set_proc_topology,
+    src/pot3d.F90:1386-1530): among the factorisations fr*ft*fp = nproc, in
+    the order fp outer, ft, fr inner, the first one minimising
+    (nr/fr - nt/ft)^2 + (nt/ft - np/fp)^2 + (np/fp - nr/fr)^2 with at least 4
+    points per rank in r and theta and 3 in phi."""
+    facs = [f for f in range(1, nproc + 1) if nproc % f == 0]
+    best, best_pen = None, None
+    for fp in facs:
+        for ft in facs:
+            for fr in facs:
+                if fr * ft * fp != nproc:
+                    continue
+                a, b, c = nr / fr, nt / ft, np_ / fp
+                pen = (a - b) ** 2 + (b - c) ** 2 + (c - a) ** 2
+                if a < 4 or b < 4 or c < 3:
+                    continue
+                if best_pen is None or pen < best_pen:
+                    best, best_pen = (fr, ft, fp), pen
+    if best is None:
+        raise ValueError(f"no rank topology for {nproc} ranks on {nr}x{nt}x{np_}")
+    return best
+
+
+def decompose_dimension(nx: int, nproc: int):
+    """Interior points per rank along one axis (decompose_dimension,
+    src/pot3d.F90:2116-2167): nx-2 points as equal as possible, the first
+    ranks taking one more."""
+    m = nx - 2
+    av, rem = divmod(m, nproc)
+    return [av + 1 if q < rem else av for q in range(nproc)]
 
 
 def _d(a):
@@ -239,6 +275,31 @@ class Oracle:
         return y
 
     # -- solve --------------------------------------------------------------
+    def cgsolve_ranks(self, rhs: np.ndarray, nprocs, epscg: float = 1e-9, stop_mode: int = 1,
+                      ncgmax: int = 1000000, x0: np.ndarray | None = None):
+        """The reference's CPU ifprec=2 solve on nprocs=(fr, ft, fp) MPI
+        ranks, one OpenMP thread per rank: per-rank diacsr/ilu0/lusol
+        (block-Jacobi ILU0), global operator and dots.  stop_mode 1 stops on
+        ||r||2/||b||2, 0 on the reference's epsn.  Returns dict(x, r, ierr,
+        ncg, epsn, t_setup, t_iter, blocks)."""
+        nb = np.array(nprocs, dtype=np.int32)
+        los = []
+        for n, f in zip((self.nr, self.nt, self.np), nprocs):
+            los.append(np.concatenate([[0], np.cumsum(decompose_dimension(n, int(f)))])
+                       .astype(np.int32))
+        x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        r = np.array(rhs, dtype=np.float64)
+        ncg = C.c_int(0)
+        epsn = C.c_double(0.0)
+        t = np.zeros(2)
+        ierr = lib().orc_cgsolve_ranks(self.gp, _d(self.coefs().ravel()), _i(nb), _i(los[0]),
+                                       _i(los[1]), _i(los[2]), int(stop_mode), float(epscg),
+                                       int(ncgmax), _d(x), _d(r), C.byref(ncg), C.byref(epsn),
+                                       _d(t))
+        return dict(x=x, r=r, ierr=ierr, ncg=ncg.value, epsn=epsn.value, t_setup=float(t[0]),
+                    t_iter=float(t[1]), blocks=[list(map(int, lo)) for lo in los])
+
+
     def cgsolve(self, rhs: np.ndarray, ifprec: int = 1, epscg: float = 1e-9,
                 ncgmax: int = 1000000, x0: np.ndarray | None = None, nhist: int = 0):
         """Returns dict(x, r, ierr, ncg, epsn, hist)."""

@@ -705,3 +705,233 @@ void orc_phi_energy(const orc_grid *g, const double *phi, double *w) {
   free(bt);
   free(bp);
 }
+
+/* ---------------------------------------------------------------------- */
+/* The reference's CPU solve with ifprec=2 on P MPI ranks, restated with
+ * one OpenMP thread per rank (shared memory in place of messages):
+ *  - rank topology nb[0] x nb[1] x nb[2] and each rank's interior index
+ *    ranges: set_proc_topology (src/pot3d.F90:1386-1530) and
+ *    decompose_dimension (:2116-2167), computed by the caller; lo[d] holds
+ *    nb[d]+1 block starts (0-based interior indices, lo[d][nb[d]] = extent);
+ *  - ax on the whole grid with the stored coefficients (orc_ax), the CG
+ *    vector updates and the dot products over all ranks (:4231-4369);
+ *  - prec_inv with ifprec=2 per rank (:5202-5220): each rank's diacsr
+ *    drops the couplings at its local r and theta boundaries, and its phi
+ *    ones when nproc_p > 1 (:4940-4967), then ilu0 + lu2luopt
+ *    (:4684-4709) and lusol on an FP32 copy of the rank's part of x.
+ * stop_mode 0: epsn < epscg (ernorm, :4432-4433); 1: ||r||2/||b||2 of the
+ * recurrence residual < epscg.  x: guess in, solution out; r: rhs in,
+ * residual out.  t[0] = preconditioner set-up seconds, t[1] = iteration
+ * seconds (both wall clock).  Returns ierr (0 converged, 1 ncgmax, 2 a
+ * rank's ilu0 found a zero pivot).                                       */
+#include <omp.h>
+
+typedef struct {
+  int i0, ni, j0, nj, k0, nk, drop_k;
+  int64_t nloc;
+  float *lu, *lud_i, *x32;
+  int *luja, *n1, *n2;
+  int icode;
+} orc_rank;
+
+/* diacsr of one rank's block (:4870-5018) in its local numbering. */
+static int rank_setup(const orc_grid *g, const double *a, orc_rank *R) {
+  const int A = g->nr - 2, B = g->nt - 2;
+  const int64_t N = (int64_t)A * B * (g->np - 2);
+  const int ni = R->ni, nj = R->nj, nk = R->nk;
+  const int64_t nl = (int64_t)ni * nj * nk, ab = (int64_t)ni * nj;
+  const int64_t ioff[7] = {-ab, -ni, -1, 0, 1, ni, ab};
+  int *IA = (int *)malloc((size_t)(nl + 1) * sizeof(int));
+  IA[0] = 1;
+  int64_t p = 0;
+  for (int mk = 0; mk < nk; mk++)
+    for (int mj = 0; mj < nj; mj++)
+      for (int mi = 0; mi < ni; mi++, p++) {
+        int cnt = 7;
+        if (mi == 0) cnt--;
+        if (mi == ni - 1) cnt--;
+        if (mj == 0) cnt--;
+        if (mj == nj - 1) cnt--;
+        if (R->drop_k && mk == 0) cnt--;
+        if (R->drop_k && mk == nk - 1) cnt--;
+        IA[p + 1] = IA[p] + cnt;
+      }
+  const int64_t M = IA[nl];
+  float *Av = (float *)malloc((size_t)M * sizeof(float));
+  int *JA = (int *)malloc((size_t)M * sizeof(int));
+  int *dptr = (int *)malloc((size_t)nl * sizeof(int));
+  p = 0;
+  for (int mk = 0; mk < nk; mk++)
+    for (int mj = 0; mj < nj; mj++)
+      for (int mi = 0; mi < ni; mi++, p++) {
+        const int64_t i = p + 1; /* 1-based local row */
+        const int64_t ng = (int64_t)(R->i0 + mi) + (int64_t)A * ((R->j0 + mj) + (int64_t)B * (R->k0 + mk));
+        int ok[7] = {1, 1, 1, 1, 1, 1, 1};
+        if (mi == 0) ok[2] = 0;
+        if (mi == ni - 1) ok[4] = 0;
+        if (mj == 0) ok[1] = 0;
+        if (mj == nj - 1) ok[5] = 0;
+        if (R->drop_k && mk == 0) ok[0] = 0;
+        if (R->drop_k && mk == nk - 1) ok[6] = 0;
+        int64_t ko = IA[i - 1];
+        for (int pass = 0; pass < 3; pass++) /* right wrap, in range, left wrap */
+          for (int jj = 0; jj < 7; jj++) {
+            if (!ok[jj]) continue;
+            int64_t j = i + ioff[jj];
+            const int where = j > nl ? 0 : (j >= 1 ? 1 : 2);
+            if (where != pass) continue;
+            if (pass == 0) j -= nl;
+            if (pass == 2) j += nl;
+            if (pass == 1 && jj == 3) dptr[i - 1] = (int)ko;
+            Av[ko - 1] = (float)a[jj * N + ng];
+            JA[ko - 1] = (int)j;
+            ko++;
+          }
+      }
+  R->nloc = nl;
+  R->icode = orc_ilu0(nl, Av, JA, IA, dptr);
+  R->lu = (float *)malloc((size_t)M * sizeof(float));
+  R->luja = (int *)malloc((size_t)M * sizeof(int));
+  R->n1 = (int *)malloc((size_t)nl * sizeof(int));
+  R->n2 = (int *)malloc((size_t)nl * sizeof(int));
+  R->lud_i = (float *)malloc((size_t)nl * sizeof(float));
+  R->x32 = (float *)malloc((size_t)nl * sizeof(float));
+  orc_lu2luopt(nl, R->lu, Av, IA, JA, R->luja, dptr, R->n1, R->n2);
+  for (int64_t n = 0; n < nl; n++) R->lud_i[n] = (float)(1.0 / (double)Av[dptr[n] - 1]);
+  free(IA);
+  free(Av);
+  free(JA);
+  free(dptr);
+  return R->icode;
+}
+
+/* prec_inv of one rank: x_32 = real(x), lusol, x = x_32 on its points. */
+static void rank_apply(const orc_grid *g, orc_rank *R, double *x) {
+  const int A = g->nr - 2, B = g->nt - 2;
+  int64_t l = 0;
+  for (int mk = 0; mk < R->nk; mk++)
+    for (int mj = 0; mj < R->nj; mj++) {
+      const double *xs = x + (R->i0 + (int64_t)A * ((R->j0 + mj) + (int64_t)B * (R->k0 + mk)));
+      for (int mi = 0; mi < R->ni; mi++) R->x32[l++] = (float)xs[mi];
+    }
+  orc_lusol(R->nloc, R->x32, R->lu, R->luja, R->n1, R->n2, R->lud_i);
+  l = 0;
+  for (int mk = 0; mk < R->nk; mk++)
+    for (int mj = 0; mj < R->nj; mj++) {
+      double *xs = x + (R->i0 + (int64_t)A * ((R->j0 + mj) + (int64_t)B * (R->k0 + mk)));
+      for (int mi = 0; mi < R->ni; mi++) xs[mi] = (double)R->x32[l++];
+    }
+}
+
+static double par_dot(const double *x, const double *y, int64_t n) {
+  double s = 0.0;
+  int64_t i;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+int orc_cgsolve_ranks(const orc_grid *g, const double *a, const int *nb, const int *lo_r,
+                      const int *lo_t, const int *lo_p, int stop_mode, double epscg, int ncgmax,
+                      double *x, double *r, int *ncg_out, double *epsn_out, double *t) {
+  const int64_t N = (int64_t)(g->nr - 2) * (g->nt - 2) * (g->np - 2);
+  const size_t tot = (size_t)g->nr * g->nt * g->np;
+  const int P = nb[0] * nb[1] * nb[2];
+  omp_set_num_threads(P);
+  double t0 = omp_get_wtime();
+  orc_rank *R = (orc_rank *)calloc((size_t)P, sizeof(orc_rank));
+  int bad = 0, q;
+#pragma omp parallel for schedule(static, 1) reduction(| : bad)
+  for (q = 0; q < P; q++) {
+    const int br = q % nb[0], bt = (q / nb[0]) % nb[1], bp = q / (nb[0] * nb[1]);
+    R[q].i0 = lo_r[br];
+    R[q].ni = lo_r[br + 1] - lo_r[br];
+    R[q].j0 = lo_t[bt];
+    R[q].nj = lo_t[bt + 1] - lo_t[bt];
+    R[q].k0 = lo_p[bp];
+    R[q].nk = lo_p[bp + 1] - lo_p[bp];
+    R[q].drop_k = nb[2] > 1;
+    if (rank_setup(g, a, &R[q])) bad = 1;
+  }
+  double *p = (double *)malloc((size_t)N * sizeof(double));
+  double *ap = (double *)malloc((size_t)N * sizeof(double));
+  double *xa = (double *)calloc(tot, sizeof(double));
+  double t1 = omp_get_wtime();
+  t[0] = t1 - t0;
+  int ncg = 0, ierr = 1;
+  double epsn = 0.0, rdotr, bdotb, bb = 0.0, rr;
+  int64_t n;
+#define PREC(v)                                             \
+  do {                                                      \
+    _Pragma("omp parallel for schedule(static, 1)")         \
+    for (q = 0; q < P; q++) rank_apply(g, &R[q], (v));      \
+  } while (0)
+  if (bad) {
+    ierr = 2;
+    goto done;
+  }
+  memcpy(p, r, (size_t)N * sizeof(double));
+  PREC(p);
+  bdotb = par_dot(r, p, N);
+  bb = par_dot(r, r, N);
+  if (bdotb == 0.0) {
+    memset(x, 0, (size_t)N * sizeof(double));
+    ierr = 0;
+    goto done;
+  }
+  orc_ax(g, a, x, ap, xa);
+#pragma omp parallel for schedule(static)
+  for (n = 0; n < N; n++) {
+    r[n] = r[n] - ap[n];
+    p[n] = r[n];
+  }
+  PREC(p);
+  rdotr = par_dot(r, p, N);
+  rr = par_dot(r, r, N);
+  for (;;) {
+    epsn = ernorm(bdotb, rdotr);
+    const double crit = stop_mode ? sqrt(rr / bb) : epsn;
+    if (crit < epscg) {
+      ierr = 0;
+      break;
+    }
+    if (ncg >= ncgmax) {
+      ierr = 1;
+      break;
+    }
+    ncg++;
+    orc_ax(g, a, p, ap, xa);
+    const double alphai = rdotr / par_dot(p, ap, N);
+#pragma omp parallel for schedule(static)
+    for (n = 0; n < N; n++) {
+      x[n] = x[n] + alphai * p[n];
+      r[n] = r[n] - alphai * ap[n];
+      ap[n] = r[n];
+    }
+    PREC(ap);
+    const double rdotr_old = rdotr;
+    rdotr = par_dot(r, ap, N);
+    if (stop_mode) rr = par_dot(r, r, N);
+    const double betai = rdotr / rdotr_old;
+#pragma omp parallel for schedule(static)
+    for (n = 0; n < N; n++) p[n] = betai * p[n] + ap[n];
+  }
+#undef PREC
+done:
+  t[1] = omp_get_wtime() - t1;
+  for (q = 0; q < P; q++) {
+    free(R[q].lu);
+    free(R[q].lud_i);
+    free(R[q].x32);
+    free(R[q].luja);
+    free(R[q].n1);
+    free(R[q].n2);
+  }
+  free(R);
+  free(p);
+  free(ap);
+  free(xa);
+  *ncg_out = ncg;
+  *epsn_out = epsn;
+  return ierr;
+}

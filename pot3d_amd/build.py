@@ -27,7 +27,8 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     """Compile every HIP source into pot3d_amd/libpot3d_hip.so; returns its path."""
-    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = ([HIPCC, *FLAGS, "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES] +
+           ["-lhsa-runtime64"])
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -865,9 +865,12 @@ __device__ __forceinline__ bool level_point(const Grid3& g, int L, int kmin, int
   return i >= 0 && i < g.a;
 }
 
+// bad (may be null): the smallest point index whose pivot is zero, not
+// finite, or whose FP32 inverse overflows (left at 0xffffffff if none).
 template <typename CP>
 __global__ void k_dilu_factor_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks bl,
-                                    double* __restrict__ d, float* __restrict__ dinv) {
+                                    double* __restrict__ d, float* __restrict__ dinv,
+                                    unsigned* __restrict__ bad) {
   int i, j, k;
   if (!level_point(g, L, kmin, i, j, k)) return;
   const uint32_t n = (uint32_t)i + (uint32_t)g.a * ((uint32_t)j + (uint32_t)g.b * (uint32_t)k);
@@ -883,7 +886,9 @@ __global__ void k_dilu_factor_level(CP cp, Grid3 g, int L, int kmin, int wrap, I
   if (bl.lo_j(j)) dp = dp - aj / d[n - (uint32_t)g.a] * tj;
   if (bl.lo_i(i)) dp = dp - ai / d[n - 1] * ti;
   d[n] = dp;
-  dinv[n] = (float)(1.0 / dp);
+  const float di = (float)(1.0 / dp);
+  dinv[n] = di;
+  if (bad && !(dp != 0.0 && isfinite(dp) && isfinite(di))) atomicMin(bad, n);
 }
 
 static void level_grid(const Grid3& g, int L, dim3& grid, int& kmin) {
@@ -899,21 +904,21 @@ static IluBlocks no_blocks(const Grid3& g, bool wrap) {
 
 template <typename CP>
 static void dilu_factor_levels(const CP& cp, const Grid3& g, const IluBlocks& bl, double* d,
-                               float* dinv, hipStream_t s) {
+                               float* dinv, unsigned* bad, hipStream_t s) {
   const int nlev = g.a + g.b + g.c - 2;
   for (int L = 0; L < nlev; L++) {
     dim3 grid;
     int kmin;
     level_grid(g, L, grid, kmin);
-    k_dilu_factor_level<CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, bl.wrap, bl, d, dinv);
+    k_dilu_factor_level<CP><<<grid, 64, 0, s>>>(cp, g, L, kmin, bl.wrap, bl, d, dinv, bad);
   }
 }
 
 void launch_dilu_factor(const Metric& m, const Grid3& g, const IluBlocks& bl, double* d,
-                        float* dinv, hipStream_t s) {
+                        float* dinv, unsigned* bad, hipStream_t s) {
   MFCoefs cp;
   cp.m = m;
-  dilu_factor_levels(cp, g, bl, d, dinv, s);
+  dilu_factor_levels(cp, g, bl, d, dinv, bad, s);
 }
 
 void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid3& g, bool wrap,
@@ -925,7 +930,7 @@ void launch_dilu_factor_stored(const CoefArrays& ca, const float* dg, const Grid
   cp.b = g.b;
   cp.c = g.c;
   cp.ab = g.ab;
-  dilu_factor_levels(cp, g, no_blocks(g, wrap), d, dinv, s);
+  dilu_factor_levels(cp, g, no_blocks(g, wrap), d, dinv, nullptr, s);
 }
 
 // Forward sweep on one hyperplane:

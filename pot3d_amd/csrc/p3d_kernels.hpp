@@ -183,9 +183,11 @@ void launch_rhs(const Metric& m, const Grid3& g, const double* br0, const double
                 double dr1, double drn, int nt, double* rhs, hipStream_t s);
 
 // D-ILU pivots (FP64, hyperplane level launches) -> dinv (FP32), with the
-// block-Jacobi cut bl.
+// block-Jacobi cut bl.  bad (may be null, preset to 0xffffffff) receives the
+// smallest point index whose pivot is zero or not finite or whose FP32
+// inverse overflows.
 void launch_dilu_factor(const Metric& m, const Grid3& g, const IluBlocks& bl, double* d,
-                        float* dinv, hipStream_t s);
+                        float* dinv, unsigned* bad, hipStream_t s);
 
 // Stored couplings of the legacy CSR input (FP32, N each unless noted):
 // lo_k/lo_j/lo_i couple p to (i,j,k-1)/(i,j-1,k)/(i-1,j,k), up_i/up_j/up_k

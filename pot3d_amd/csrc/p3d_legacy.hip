@@ -9,11 +9,15 @@
 // planes and factorised as D-ILU (FP64 pivots), which equals ILU(0) on this
 // pattern.  Anything else is rejected with exit(1), as the reference shim
 // does on any sparse-library error.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <vector>
+
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 
 #include "../../include/pot3d_hip.h"
 #include "p3d_kernels.hpp"
@@ -60,33 +64,92 @@ void check(hipError_t e, const char* what) {
   }
 }
 
-bool is_device_ptr(const void* p) {
+// Where a caller address lives.
+//   Hip:  device (or managed) memory that HIP tracks: hipMemcpy and kernels.
+//   Hsa:  an allocation of the HSA runtime that HIP does not track, such as
+//         the device arrays of an OpenMP-offload program (libomptarget
+//         allocates from the HSA memory pools; the -DCUSPARSE calling
+//         convention passes them through use_device_addr, src/pot3d.F90:
+//         4675-4683): reachable by kernels of this process only, so copies
+//         go through a copy kernel.
+//   Host: plain, pinned or registered host memory: memcpy.
+enum Where { W_HOST = 0, W_HIP = 1, W_HSA = 2 };
+
+Where classify(const void* p) {
   hipPointerAttribute_t at;
-  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
+  if (hipPointerGetAttributes(&at, p) == hipSuccess) {
+    if (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged) return W_HIP;
+    return W_HOST;
   }
-  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+  (void)hipGetLastError();
+  hsa_amd_pointer_info_t info;
+  memset(&info, 0, sizeof info);
+  info.size = sizeof info;
+  if (hsa_amd_pointer_info(p, &info, nullptr, nullptr, nullptr) == HSA_STATUS_SUCCESS &&
+      (info.type == HSA_EXT_POINTER_TYPE_HSA || info.type == HSA_EXT_POINTER_TYPE_IPC ||
+       info.type == HSA_EXT_POINTER_TYPE_HSA_VMEM))
+    return W_HSA;
+  return W_HOST;
 }
 
-// Resolve a caller pointer: an OpenMP-mapped host address -> its device
-// address; a device address -> itself; else nullptr (plain host memory).
-void* device_view(void* p) {
+struct View {
+  void* p;
+  Where w;
+};
+
+// Resolves a caller pointer: an OpenMP-mapped host address (the
+// -DHIPSPARSE convention, src/lusol_hipsparse.c:75-79) -> its device
+// address; anything else is classified as it is.
+View view(void* p) {
   if (omp_get_mapped_ptr && omp_get_default_device) {
     void* d = omp_get_mapped_ptr(p, omp_get_default_device());
-    if (d) return d;
+    if (d) return View{d, classify(d)};
   }
-  return is_device_ptr(p) ? p : nullptr;
+  return View{p, classify(p)};
+}
+
+__global__ void k_copy_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Copies nbytes (a multiple of 4) between a caller buffer v and host memory
+// h, in the direction to_host.
+void move(View v, void* h, size_t nbytes, bool to_host) {
+  if (v.w == W_HOST) {
+    if (to_host)
+      memcpy(h, v.p, nbytes);
+    else
+      memcpy(v.p, h, nbytes);
+    return;
+  }
+  const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+  if (v.w == W_HIP) {
+    check(to_host ? hipMemcpy(h, v.p, nbytes, kind) : hipMemcpy(v.p, h, nbytes, kind),
+          "copy CSR");
+    return;
+  }
+  void* stage = nullptr;
+  const size_t nw = nbytes / 4;
+  const unsigned blocks = (unsigned)((nw + 255) / 256);
+  check(hipMalloc(&stage, nbytes ? nbytes : 4), "allocate staging");
+  if (to_host) {
+    if (nw) k_copy_words<<<blocks, 256, 0, G.stream>>>((const uint32_t*)v.p, (uint32_t*)stage, nw);
+    check(hipStreamSynchronize(G.stream), "copy from HSA memory");
+    check(hipMemcpy(h, stage, nbytes, hipMemcpyDeviceToHost), "copy CSR");
+  } else {
+    check(hipMemcpy(stage, h, nbytes, hipMemcpyHostToDevice), "copy CSR");
+    if (nw) k_copy_words<<<blocks, 256, 0, G.stream>>>((const uint32_t*)stage, (uint32_t*)v.p, nw);
+    check(hipStreamSynchronize(G.stream), "copy to HSA memory");
+  }
+  hipFree(stage);
 }
 
 template <typename V>
 void fetch(std::vector<V>& dst, const V* src, size_t n) {
   dst.resize(n);
-  void* dv = device_view((void*)src);
-  if (dv)
-    check(hipMemcpy(dst.data(), dv, n * sizeof(V), hipMemcpyDeviceToHost), "copy CSR to host");
-  else
-    memcpy(dst.data(), src, n * sizeof(V));
+  move(view((void*)src), dst.data(), n * sizeof(V), true);
 }
 
 // Expected columns (1-based, in diacsr order) of row p=(i,j,k).
@@ -261,8 +324,9 @@ void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M
   check(hipMemcpy(d.data(), dpiv, n * sizeof(double), hipMemcpyDeviceToHost), "copy pivots");
   hipFree(dpiv);
   for (size_t p = 0; p < n; p++)
-    if (d[p] == 0.0) {
-      fprintf(stderr, " ERROR! ILU0 zero pivot at row %zu\n", p + 1);
+    if (d[p] == 0.0 || !isfinite(d[p])) {
+      fprintf(stderr, " ERROR! ILU0 zero or non-finite pivot %g at row %zu\n", d[p], p + 1);
+      fflush(stderr);
       exit(1);
     }
   for (size_t p = 0; p < n; p++) {
@@ -274,11 +338,7 @@ void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M
         AV[e] = (float)((double)AV[e] / d[q]);
     }
   }
-  void* dv = device_view(csr_lu);
-  if (dv)
-    check(hipMemcpy(dv, AV.data(), nnz * sizeof(float), hipMemcpyHostToDevice), "write LU");
-  else
-    memcpy(csr_lu, AV.data(), nnz * sizeof(float));
+  move(view(csr_lu), AV.data(), nnz * sizeof(float), false);
   int dev = 0, ncu = 256;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
@@ -289,7 +349,8 @@ void load_external_lu_solver(float* csr_lu, int* csr_i, int* csr_j, int N, int M
 
 void external_lu_solver(double* x) {
   if (!G.loaded) die("external_lu_solver called before load_external_lu_solver");
-  double* dx = (double*)device_view(x);
+  const View xv = view(x);
+  double* dx = xv.w == W_HOST ? nullptr : (double*)xv.p;  // kernels work in place
   const size_t nb = (size_t)G.N * sizeof(double);
   if (!dx) {
     if (!G.xbuf) check(hipMalloc((void**)&G.xbuf, nb), "allocate x");

@@ -206,10 +206,6 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
     const CgState* gd = guard ? s->st : nullptr;
     if (s->sweep_mode == 0 && s->sweep_ready) {
       launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
-    } else if (s->sweep_mode == 2 && s->sweep_ready && !guard && !s->sweep.blocked) {
-      // persistent dataflow schedule: single applications only
-      launch_dilu_sweep<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, nullptr, nullptr,
-                           s->stream);
     } else {
       launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, s->blocks, gd,
                                   s->stream);
@@ -556,10 +552,6 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
 // Core solve on the solver's own buffers: s->x guess, s->b rhs.
 static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
   int rc;
-  if (s->ifprec == PREC_ILU && s->sweep_mode == 2)
-    return fail(P3D_EINVAL,
-                "the persistent dataflow ILU sweep (P3D_OPT_SWEEP=2) is disabled inside the CG "
-                "iteration; use 0 (tile diagonals) or 1 (hyperplanes)");
   auto t0 = std::chrono::steady_clock::now();
   memset(&s->stats, 0, sizeof s->stats);
   if (s->guess_check) {
@@ -611,6 +603,46 @@ static int solve_core(p3d_solver* s, double epscg, int ncgmax, int ncghist) {
   auto t1 = std::chrono::steady_clock::now();
   s->stats.t_solve = std::chrono::duration<double>(t1 - t0).count();
   return rc;
+}
+
+// D-ILU factorisation (FP64 pivots, FP32 inverses) of the current
+// block-Jacobi cut.  A zero or non-finite pivot, or one whose FP32 inverse
+// overflows, fails with P3D_ENONFINITE: the reference's host ilu0 prints a
+// zero pivot and goes on (src/pot3d.F90:4695-4697), which would leave the
+// iteration running on inf/NaN.
+static int factor_ilu(p3d_solver* s) {
+  double* dpiv = nullptr;
+  unsigned* bad = nullptr;
+  int rc = dalloc((void**)&dpiv, sizeof(double) * s->g.N);
+  if (rc) return rc;
+  if ((rc = dalloc((void**)&bad, sizeof(unsigned)))) {
+    hipFree(dpiv);
+    return rc;
+  }
+  unsigned first = 0xffffffffu;
+  double dv = 0.0;
+  hipError_t e = hipMemcpyAsync(bad, &first, sizeof first, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) {
+    launch_dilu_factor(s->m, s->g, s->blocks, dpiv, s->dinv, bad, s->stream);
+    e = hipStreamSynchronize(s->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpy(&first, bad, sizeof first, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && first != 0xffffffffu)
+    e = hipMemcpy(&dv, dpiv + first, sizeof dv, hipMemcpyDeviceToHost);
+  hipFree(dpiv);
+  hipFree(bad);
+  if (e != hipSuccess)
+    return fail(P3D_EHIP, "D-ILU factorisation failed: %s", hipGetErrorString(e));
+  if (first != 0xffffffffu) {
+    const int a = s->g.a, b = s->g.b;
+    const int i = (int)(first % (unsigned)a), j = (int)((first / (unsigned)a) % (unsigned)b);
+    const int k = (int)(first / ((unsigned)a * (unsigned)b));
+    return fail(P3D_ENONFINITE,
+                "ILU0 pivot %g at row %u (padded point (%d,%d,%d)): zero, not finite or "
+                "beyond FP32 range",
+                dv, first + 1, i + 2, j + 2, k + 2);
+  }
+  return P3D_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -809,15 +841,11 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   hipMemset(s->pole_part, 0, sizeof(double) * 2 * kPoleSegs * a);
   if (ifprec == 2) {
     auto t0 = std::chrono::steady_clock::now();
-    double* dpiv = nullptr;
     if ((rc = dalloc((void**)&s->yv, N * sizeof(float))) ||
-        (rc = dalloc((void**)&s->dinv, N * sizeof(float))) || (rc = dalloc((void**)&dpiv, nb)))
+        (rc = dalloc((void**)&s->dinv, N * sizeof(float))))
       return bail(rc);
     s->blocks = IluBlocks{s->g.a, s->g.b, s->g.c, 1};
-    launch_dilu_factor(s->m, s->g, s->blocks, dpiv, s->dinv, s->stream);
-    hipStreamSynchronize(s->stream);
-    hipFree(dpiv);
-    if (hipGetLastError() != hipSuccess) return bail(fail(P3D_EHIP, "D-ILU factorisation failed"));
+    if ((rc = factor_ilu(s))) return bail(rc);
     s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
                                        &s->blocks, s->dinv, s->tile_stage) == 0);
     if ((rc = update_tilemap(s))) return bail(rc);
@@ -857,13 +885,8 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
   }
   if (refactor) {
     s->blocks = ilu_blocks(s);
-    double* dpiv = nullptr;
-    int rc = dalloc((void**)&dpiv, sizeof(double) * s->g.N);
+    const int rc = factor_ilu(s);
     if (rc) return rc;
-    launch_dilu_factor(s->m, s->g, s->blocks, dpiv, s->dinv, s->stream);
-    hipError_t e = hipStreamSynchronize(s->stream);
-    hipFree(dpiv);
-    if (e != hipSuccess) return fail(P3D_EHIP, "D-ILU factorisation failed");
   }
   int dev = 0, ncu = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -901,8 +924,8 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       s->use_graph = value != 0;
       break;
     case P3D_OPT_SWEEP:
-      if (value != 0 && value != 1 && value != 2)
-        return fail(P3D_EINVAL, "P3D_OPT_SWEEP must be 0, 1 or 2");
+      if (value != 0 && value != 1)
+        return fail(P3D_EINVAL, "P3D_OPT_SWEEP must be 0 or 1");
       s->sweep_mode = (int)value;
       for (int t = 0; t < 3; t++) {
         if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
@@ -1145,8 +1168,6 @@ int p3d_apply_prec(p3d_solver* s, double* x) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(x, s->ap, nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (s->ifprec == PREC_ILU && sweep_error(s->sweep))
-    return fail(P3D_EHIP, "ILU dataflow sweep exceeded its wait bound (results invalid)");
   return P3D_OK;
 }
 

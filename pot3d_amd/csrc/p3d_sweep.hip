@@ -1,4 +1,4 @@
-// Dataflow D-ILU sweeps (see p3d_sweep.hpp for the scheme).
+// D-ILU triangular solves on the tile-diagonal schedule (see p3d_sweep.hpp).
 //
 // Arithmetic per point is identical to the hyperplane version in
 // p3d_kernels.hip (same coefficients, same operation order):
@@ -6,14 +6,6 @@
 //            t_q = dinv_q y_q
 //   backward z_p = (y_p - bi z_(i+1) - bj z_(j+1) - bk z_(k+1) - bw z_w) dinv_p
 // so both schedules give bit-identical results.
-//
-// Memory ordering between waves (message passing at agent scope, the
-// AMDGPU memory model's pattern for multi-XCD parts whose L2s are not
-// coherent with each other): a tile publishes progress with an agent-scope
-// release fence (its stores are written back from its XCD's L2) followed
-// by the counter store; a waiting tile polls the counter with relaxed
-// agent-scope loads and, once satisfied, issues an agent-scope acquire
-// fence (stale lines in its XCD's L2 are dropped) before reading faces.
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -31,55 +23,9 @@ constexpr int SKEW = TI + TJ - 2;  // steps between the first and last lane of a
 // 32-plane tiles measured no faster)
 constexpr int kTileRound = 16;
 constexpr int U = 16;              // steps per prefetch block
-constexpr unsigned kSpinMax = 1u << 20;
 }  // namespace
 constexpr int kTileK = 32;         // default planes per tile of the tile-diagonal sweep
 namespace {
-
-__device__ __forceinline__ unsigned ld_u32(const unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_u32(unsigned* p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Face values are ordered by the release/acquire fences around the
-// progress counters; the accesses themselves are plain loads and stores.
-__device__ __forceinline__ float ld_coh(const float* p) { return *p; }
-__device__ __forceinline__ void st_coh(float* p, float v) { *p = v; }
-
-// Wave-uniform bounded wait until prog[item] >= need.
-__device__ __forceinline__ void wait_progress(const unsigned* prog, int item, unsigned need,
-                                              unsigned& known, unsigned* err) {
-  if (known >= need) return;
-  if (ld_u32(err)) {  // an earlier wait failed: finish quickly, results are void
-    known = need;
-    return;
-  }
-  unsigned v = ld_u32(prog + item);
-  unsigned spins = 0;
-  while (v < need) {
-    __builtin_amdgcn_s_sleep(2);
-    if (++spins > kSpinMax) {
-      if (threadIdx.x == 0) st_u32(err, 1u);
-      v = need;
-      break;
-    }
-    v = ld_u32(prog + item);
-  }
-  known = v;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
-__device__ __forceinline__ void publish(unsigned* prog, int item, int planes) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  if (threadIdx.x == 0) st_u32(prog + item, (unsigned)planes);
-}
-
-__device__ __forceinline__ unsigned take_ticket(unsigned* ctr) {
-  unsigned t = 0;
-  if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __shfl(t, 0, 64);
-}
 
 // Neighbour exchange inside one wave holding an 8x8 block of columns, lane
 // l = ii + 8 jj.  lanes_from_lower gives lane l the values of lanes l-1 and
@@ -107,232 +53,14 @@ __device__ __forceinline__ void lanes_from_upper(float t, float& from_i, float& 
   from_j = __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + 8) & 63) << 2, v));
 }
 
-// Last wave out re-arms the ticket and progress counters for the next launch.
-__device__ void sweep_exit(unsigned* ticket, unsigned* exited, unsigned* prog, int nitems) {
-  unsigned last = 0;
-  if (threadIdx.x == 0)
-    last = (__hip_atomic_fetch_add(exited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-            gridDim.x - 1);
-  last = __shfl(last, 0, 64);
-  if (!last) return;
-  for (int q = threadIdx.x; q < nitems; q += 64) st_u32(prog + q, 0u);
-  if (threadIdx.x == 0) {
-    st_u32(ticket, 0u);
-    st_u32(exited, 0u);
-  }
-}
-
-template <typename T>
-__device__ void fwd_item(const Metric& m, const Grid3& g, int I, int J, int item, int NI,
-                         const float* __restrict__ dinv, const T* __restrict__ r,
-                         float* __restrict__ yv, unsigned* prog, unsigned* err) {
-  const int lane = threadIdx.x;
-  const int ii = lane & (TI - 1), jj = lane / TI;
-  const int i = I * TI + ii, j = J * TJ + jj;
-  const bool act = (i < g.a) && (j < g.b);
-  const int ic = act ? i : 0, jc = act ? j : 0;
-  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float aw = (float)(-(P1 * m.DPU[g.c - 1]));
-  const int left = (I > 0) ? item - 1 : -1;
-  const int down = (J > 0) ? item - NI : -1;
-  unsigned known_l = 0, known_d = 0;
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  const int S = g.c + SKEW;
-  for (int s0 = 0; s0 < S; s0 += U) {
-    const unsigned need = (unsigned)min(g.c, s0 + U);
-    if (left >= 0) wait_progress(prog, left, need, known_l, err);
-    if (down >= 0) wait_progress(prog, down, need, known_d, err);
-    float rr[U], dd[U], fl[U], fd[U], ak[U], aj[U], ai[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int k = s0 + u - ii - jj;
-      rr[u] = dd[u] = fl[u] = fd[u] = ak[u] = aj[u] = ai[u] = 0.f;
-      if (act && k >= 0 && k < g.c) {
-        const uint32_t p = col + g.ab * (uint32_t)k;
-        rr[u] = (float)r[p];
-        dd[u] = dinv[p];
-        const double dph = m.DPH[k];
-        ak[u] = (float)(-(P1 * m.DPL[k]));
-        aj[u] = (float)(-((DRH * dph) * CTL));
-        ai[u] = (float)(-((SD * dph) * CRL));
-        if (ii == 0 && i > 0) fl[u] = dinv[p - 1] * ld_coh(yv + p - 1);
-        if (jj == 0 && j > 0) fd[u] = dinv[p - g.a] * ld_coh(yv + p - g.a);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int k = s0 + u - ii - jj;
-      float from_i, from_j;
-      lanes_from_lower(t_pub, from_i, from_j);
-      if (act && k >= 0 && k < g.c) {
-        float y = rr[u];
-        if (k == g.c - 1) y = y - aw * t_w;
-        if (k > 0) y = y - ak[u] * t_own;
-        if (j > 0) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-        if (i > 0) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
-        st_coh(yv + col + g.ab * (uint32_t)k, y);
-        const float t = dd[u] * y;
-        if (k == 0) t_w = t;
-        t_own = t;
-        t_pub = t;
-      }
-    }
-    publish(prog, item, min(g.c, max(0, s0 + U - SKEW)));
-  }
-}
-
-template <typename T>
-__device__ double bwd_item(const Metric& m, const Grid3& g, int I, int J, int item, int NI,
-                           int NJ, const float* __restrict__ dinv, float* __restrict__ yv,
-                           T* __restrict__ z, const T* __restrict__ rdot, unsigned* prog,
-                           unsigned* err) {
-  double acc = 0.0;
-  const int lane = threadIdx.x;
-  const int ii = lane & (TI - 1), jj = lane / TI;
-  const int rii = TI - 1 - ii, rjj = TJ - 1 - jj;
-  const int i = I * TI + ii, j = J * TJ + jj;
-  const bool act = (i < g.a) && (j < g.b);
-  const int ic = act ? i : 0, jc = act ? j : 0;
-  const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
-  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-  const double P1 = DRH * m.CP[jc];
-  const float bw = (float)(-(P1 * m.DPL[0]));
-  const int right = (I + 1 < NI) ? item + 1 : -1;
-  const int up = (J + 1 < NJ) ? item + NI : -1;
-  unsigned known_r = 0, known_u = 0;
-  float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
-  const int S = g.c + SKEW;
-  for (int s0 = 0; s0 < S; s0 += U) {
-    const unsigned need = (unsigned)min(g.c, s0 + U);
-    if (right >= 0) wait_progress(prog, right, need, known_r, err);
-    if (up >= 0) wait_progress(prog, up, need, known_u, err);
-    float yy[U], dd[U], fr[U], fu[U], bk[U], bj[U], bi[U];
-    T rd[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int k = (g.c - 1) - (s0 + u - rii - rjj);
-      yy[u] = dd[u] = fr[u] = fu[u] = bk[u] = bj[u] = bi[u] = 0.f;
-      rd[u] = (T)0;
-      if (act && k >= 0 && k < g.c) {
-        const uint32_t p = col + g.ab * (uint32_t)k;
-        yy[u] = yv[p];
-        if (rdot) rd[u] = rdot[p];
-        dd[u] = dinv[p];
-        const double dph = m.DPH[k];
-        bk[u] = (float)(-(P1 * m.DPU[k]));
-        bj[u] = (float)(-((DRH * dph) * CTH));
-        bi[u] = (float)(-((SD * dph) * CRH));
-        if (ii == TI - 1 && i < g.a - 1) fr[u] = ld_coh(yv + p + 1);
-        if (jj == TJ - 1 && j < g.b - 1) fu[u] = ld_coh(yv + p + g.a);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int k = (g.c - 1) - (s0 + u - rii - rjj);
-      float from_i, from_j;
-      lanes_from_upper(t_pub, from_i, from_j);
-      if (act && k >= 0 && k < g.c) {
-        float sv = yy[u];
-        if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-        if (j < g.b - 1) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-        if (k < g.c - 1) sv = sv - bk[u] * t_own;
-        if (k == 0) sv = sv - bw * t_w;
-        const float zz = sv * dd[u];
-        const uint32_t p = col + g.ab * (uint32_t)k;
-        st_coh(yv + p, zz);
-        z[p] = (T)zz;
-        acc += (double)rd[u] * (double)(T)zz;
-        if (k == g.c - 1) t_w = zz;
-        t_own = zz;
-        t_pub = zz;
-      }
-    }
-    publish(prog, item, min(g.c, max(0, s0 + U - SKEW)));
-  }
-  return acc;
-}
-
-struct SweepArgs {
-  int NI, NJ, nitems;
-  const int* order;
-  unsigned* prog;
-  SweepCtl* ctl;
-  double* item_partial;  // per-item r.z (backward sweep with a fused dot)
-  CgState* st;           // epilogue target of the fused dot
-};
-
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64)
-    k_sweep_fwd(Metric m, Grid3 g, SweepArgs sa, const float* __restrict__ dinv,
-                const T* __restrict__ r, float* __restrict__ yv, const CgState* guard) {
-  if (guard && st_done(guard)) return;
-  for (;;) {
-    const unsigned t = take_ticket(&sa.ctl->ticket_f);
-    if (t >= (unsigned)sa.nitems) break;
-    const int item = sa.order[t];
-    fwd_item<T>(m, g, item % sa.NI, item / sa.NI, item, sa.NI, dinv, r, yv, sa.prog,
-                &sa.ctl->error);
-  }
-  sweep_exit(&sa.ctl->ticket_f, &sa.ctl->exited_f, sa.prog, sa.nitems);
-}
-
-// Backward sweep; with rdot set, also reduces r.z: each tile stores its
-// partial, and the last wave out sums them in tile order (deterministic)
-// and runs the CG epilogue EPI_RZITER (beta, epsn, convergence test).
-template <typename T>
-__global__ void __launch_bounds__(64)
-    k_sweep_bwd(Metric m, Grid3 g, SweepArgs sa, const float* __restrict__ dinv,
-                float* __restrict__ yv, T* __restrict__ z, const T* __restrict__ rdot,
-                const CgState* guard) {
-  if (guard && st_done(guard)) return;
-  for (;;) {
-    const unsigned t = take_ticket(&sa.ctl->ticket_b);
-    if (t >= (unsigned)sa.nitems) break;
-    const int item = sa.order[t];
-    const double acc = bwd_item<T>(m, g, item % sa.NI, item / sa.NI, item, sa.NI, sa.NJ, dinv,
-                                   yv, z, rdot, sa.prog, &sa.ctl->error);
-    if (rdot) {
-      const double w = wave_sum64(acc);
-      if (threadIdx.x == 0)
-        __hip_atomic_store(sa.item_partial + item, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  unsigned last = 0;
-  if (threadIdx.x == 0)
-    last = (__hip_atomic_fetch_add(&sa.ctl->exited_b, 1u, __ATOMIC_ACQ_REL,
-                                   __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1);
-  last = __shfl(last, 0, 64);
-  if (!last) return;
-  if (rdot) {
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < sa.nitems; q += 64)
-      acc += __hip_atomic_load(sa.item_partial + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc = wave_sum64(acc);
-    if (threadIdx.x == 0) cg_epilogue(sa.st, EPI_RZITER, acc, 0.0, nullptr);
-  }
-  for (int q = threadIdx.x; q < sa.nitems; q += 64) st_u32(sa.prog + q, 0u);
-  if (threadIdx.x == 0) {
-    st_u32(&sa.ctl->ticket_b, 0u);
-    st_u32(&sa.ctl->exited_b, 0u);
-  }
-}
-
-
 // ---------------------------------------------------------------------------
 // Tile-diagonal sweeps: the (i,j,k) grid is cut into 8x8xTK tiles and one
 // launch processes every tile on one anti-diagonal I+J+K (reversed for the
 // backward sweep).  A tile's inputs from its lower neighbours were finished
 // by earlier launches, so there is no waiting between workgroups at all;
-// inside a tile the lanes march through the TK planes with the same skewed
-// schedule, shuffles and arithmetic as the dataflow sweep.
+// inside a tile the lanes march through the TK planes: lane (ii,jj) handles
+// plane k = s - ii - jj at step s, so its (i-1) and (j-1) neighbours hand
+// their values over one step earlier by cross-lane shuffle.
 // Coefficient sources of the tile sweeps: matrix-free (metric factors,
 // the solver) or stored FP32 couplings (the legacy CSR input).
 struct TileMF {
@@ -1331,42 +1059,6 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
                      const IluBlocks* blk, const float* dinv, int stage_one) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
-  plan.NI = (g.a + TI - 1) / TI;
-  plan.NJ = (g.b + TJ - 1) / TJ;
-  plan.nitems = plan.NI * plan.NJ;
-  // forward: by anti-diagonal I+J (then I); every dependency (I-1,J),
-  // (I,J-1) gets a smaller ticket.  Backward: the reverse order.
-  std::vector<int> f(plan.nitems);
-  for (int q = 0; q < plan.nitems; q++) f[q] = q;
-  const int NI = plan.NI;
-  std::stable_sort(f.begin(), f.end(), [NI](int x, int y) {
-    const int dx = x % NI + x / NI, dy = y % NI + y / NI;
-    if (dx != dy) return dx < dy;
-    return x % NI < y % NI;
-  });
-  std::vector<int> b(f.rbegin(), f.rend());
-  const size_t ni = (size_t)plan.nitems;
-  if (hipMalloc((void**)&plan.order_f, ni * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&plan.order_b, ni * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&plan.prog_f, ni * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc((void**)&plan.prog_b, ni * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc((void**)&plan.ctl, sizeof(SweepCtl)) != hipSuccess ||
-      hipMalloc((void**)&plan.item_partial, ni * sizeof(double)) != hipSuccess) {
-    free_sweep_plan(plan);
-    return 1;
-  }
-  hipMemcpy(plan.order_f, f.data(), ni * sizeof(int), hipMemcpyHostToDevice);
-  hipMemcpy(plan.order_b, b.data(), ni * sizeof(int), hipMemcpyHostToDevice);
-  hipMemset(plan.prog_f, 0, ni * sizeof(unsigned));
-  hipMemset(plan.prog_b, 0, ni * sizeof(unsigned));
-  hipMemset(plan.ctl, 0, sizeof(SweepCtl));
-  // every tile marches through all planes: keep them all resident when possible
-  int cap = 16 * (num_cus > 0 ? num_cus : 256);
-  plan.grid = plan.nitems < cap ? plan.nitems : cap;
-  if (hipDeviceSynchronize() != hipSuccess) {
-    free_sweep_plan(plan);
-    return 1;
-  }
   // tile diagonals
   plan.stage_one = stage_one ? 1 : 0;
   plan.TK = (tile_k == 16 || tile_k == 64) ? tile_k : kTileK;
@@ -1448,12 +1140,6 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
 }
 
 void free_sweep_plan(SweepPlan& plan) {
-  if (plan.order_f) hipFree(plan.order_f);
-  if (plan.order_b) hipFree(plan.order_b);
-  if (plan.prog_f) hipFree(plan.prog_f);
-  if (plan.prog_b) hipFree(plan.prog_b);
-  if (plan.ctl) hipFree(plan.ctl);
-  if (plan.item_partial) hipFree(plan.item_partial);
   if (plan.tiles) hipFree(plan.tiles);
   if (plan.face_i) hipFree(plan.face_i);
   if (plan.face_j) hipFree(plan.face_j);
@@ -1462,29 +1148,6 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.face_k) hipFree(plan.face_k);
   if (plan.face_kt) hipFree(plan.face_kt);
   plan = SweepPlan();
-}
-
-int sweep_error(const SweepPlan& plan) {
-  if (!plan.ready) return 0;
-  unsigned e = 0;
-  hipMemcpy(&e, &plan.ctl->error, sizeof(unsigned), hipMemcpyDeviceToHost);
-  if (e) {
-    unsigned zero = 0;
-    hipMemcpy(&plan.ctl->error, &zero, sizeof(unsigned), hipMemcpyHostToDevice);
-  }
-  return e ? 1 : 0;
-}
-
-template <typename T>
-void launch_dilu_sweep(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
-                       const T* r, float* yv, T* z, const CgState* guard, CgState* dot_st,
-                       hipStream_t s) {
-  SweepArgs fa{plan.NI, plan.NJ, plan.nitems, plan.order_f, plan.prog_f, plan.ctl, nullptr,
-               nullptr};
-  SweepArgs ba{plan.NI, plan.NJ, plan.nitems, plan.order_b, plan.prog_b, plan.ctl,
-               plan.item_partial, dot_st};
-  k_sweep_fwd<T><<<plan.grid, 64, 0, s>>>(m, g, fa, dinv, r, yv, guard);
-  k_sweep_bwd<T><<<plan.grid, 64, 0, s>>>(m, g, ba, dinv, yv, z, dot_st ? r : nullptr, guard);
 }
 
 template <typename T>
@@ -1507,12 +1170,5 @@ template void launch_dilu_tiles<double>(const SweepPlan&, const Metric&, const G
 template void launch_dilu_tiles<float>(const SweepPlan&, const Metric&, const Grid3&,
                                        const float*, const float*, float*, float*,
                                        const CgState*, hipStream_t);
-
-template void launch_dilu_sweep<double>(const SweepPlan&, const Metric&, const Grid3&,
-                                        const float*, const double*, float*, double*,
-                                        const CgState*, CgState*, hipStream_t);
-template void launch_dilu_sweep<float>(const SweepPlan&, const Metric&, const Grid3&,
-                                       const float*, const float*, float*, float*,
-                                       const CgState*, CgState*, hipStream_t);
 
 }  // namespace p3d

@@ -1,17 +1,14 @@
-// Dataflow D-ILU sweeps: one persistent launch per triangular solve.
+// Tile-diagonal D-ILU triangular solves.
 //
-// The (i,j) plane is cut into 8x8 tiles; one wave owns a tile (lane =
-// ii + 8*jj) and marches along k (phi) through all c planes.  Lane (ii,jj)
-// handles plane k = s - ii - jj at step s, so its (i-1) and (j-1)
-// neighbours (lanes -1 and -8) finished the same plane one step earlier
-// and hand their values over by cross-lane shuffle; (k-1) and the phi wrap
-// partner stay in the lane's registers.  Tile faces come from the tiles
-// below in i and j, which publish per-tile progress (planes finished)
-// after every block of steps.  Tiles are claimed through an atomic ticket
-// in topological order, so every tile waited on is already owned by a
-// running wave: the sweep cannot deadlock.  Waits are bounded; a wait that
-// exceeds the bound sets an error flag (the sweep then finishes with wrong
-// values and the solver reports the error) instead of hanging.
+// The (i,j,k) grid is cut into 8x8xTK tiles (or (8 SI)x(8 SJ)xTK with
+// several waves per tile); one launch processes every tile of one
+// anti-diagonal I+J+Kc, so a tile's faces come from tiles finished by
+// earlier launches and no workgroup ever waits for another.  Inside a tile
+// one wave owns an 8x8 column stack (lane = ii + 8*jj) and marches along k
+// (phi): lane (ii,jj) handles plane k = s - ii - jj at step s, so its (i-1)
+// and (j-1) neighbours finished the same plane one step earlier and hand
+// their values over by cross-lane shuffle; (k-1) and the phi wrap partner
+// stay in the lane's registers.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,23 +18,8 @@
 
 namespace p3d {
 
-struct SweepCtl {
-  unsigned ticket_f, exited_f;  // forward sweep
-  unsigned ticket_b, exited_b;  // backward sweep
-  unsigned error;               // wait bound exceeded
-  unsigned pad[3];
-};
-
 struct SweepPlan {
   int ready = 0;
-  int NI = 0, NJ = 0, nitems = 0;
-  int grid = 0;                 // workgroups (one wave each)
-  int* order_f = nullptr;       // ticket -> item (I + NI*J), forward order
-  int* order_b = nullptr;       // backward order
-  unsigned* prog_f = nullptr;   // per-item planes finished (forward)
-  unsigned* prog_b = nullptr;   // per-item planes finished (backward)
-  SweepCtl* ctl = nullptr;
-  double* item_partial = nullptr;  // per-tile partial r.z of the backward sweep
   // tile-diagonal schedule ((8 SI) x (8 SJ) x TK tiles, one launch per
   // anti-diagonal; SI x SJ waves per tile)
   int TK = 0, NK = 0, ndiag = 0;
@@ -62,7 +44,7 @@ struct SweepPlan {
   int stage_one = 0;
 };
 
-// Returns 0 when the dataflow sweep can run on this grid.
+// Returns 0 when the tile sweeps can run on this grid.
 // tile_k: planes per tile of the tile-diagonal schedule (16, 32 or 64).
 // tile_waves: waves per tile, 1 (8x8), 4 (16x16) or 8 (16x32; tile_k <= 32).
 // bl: block-Jacobi cut of the ILU (null: none); its extents must be
@@ -74,9 +56,6 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 
                      int tile_waves = 1, const IluBlocks* bl = nullptr,
                      const float* dinv = nullptr, int stage_one = 0);
 void free_sweep_plan(SweepPlan& plan);
-// 1 if a sweep wait exceeded its bound since the last call (and clears it).
-int sweep_error(const SweepPlan& plan);
-
 // Tile-diagonal sweeps: z = (LU)^-1 r with 2*ndiag launches and no waiting
 // between workgroups (the schedule used inside the CG iteration).
 template <typename T>
@@ -87,16 +66,5 @@ void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, c
 void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,
                               const float* dinv, const double* r, float* yv, double* z,
                               hipStream_t s);
-
-// Persistent dataflow sweeps (single applications only: the solve rejects
-// this schedule).
-// z = (LU)^-1 r with the matrix-free D-ILU factors (dinv = 1/pivots).
-// yv: FP32 scratch of N (forward output, overwritten by z in FP32).
-// dot_st non-null: the backward sweep also reduces r.z (tile order) and
-// runs the CG epilogue EPI_RZITER on dot_st.
-template <typename T>
-void launch_dilu_sweep(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
-                       const T* r, float* yv, T* z, const CgState* guard, CgState* dot_st,
-                       hipStream_t s);
 
 }  // namespace p3d

@@ -1,0 +1,180 @@
+"""GPU solves at the benchmark configurations' own workloads, through the C
+ABI, against the CPU oracle (where it finishes in seconds) and against the
+FP64 GPU solve (full sizes, size-independent properties).
+
+  cfg1  51x91x181 PFSS ('ss', phi=0 at r1), FP64 PCG with the diagonal
+        preconditioner (ifprec=1): against the oracle's cgsolve.
+  cfg3  the cfg1 grid with ILU0, FP32 inner PCG inside FP64 defect
+        correction (both mixed modes), and the Neumann variant ('potential',
+        dphi/dr=0 at r1, src/pot3d.F90:1250-1255): against the oracle's FP64
+        solve; the Neumann system is singular (constants), so phi is compared
+        after removing its mean.
+  cfg2  151x301x601 PFSS, ILU0: the FP64 solve's residual recomputed by the
+        oracle's operator; mixed precision with the block-Jacobi ILU0 against
+        the FP64 solve.
+  cfg5  301x601x1201 'open' (r in [2.5, 10], 16 bipoles): mixed precision
+        with the block-Jacobi ILU0 against the FP64 solve.
+
+Bars (the north-star bar at epscg = 1e-9): potential and B-field (getb,
+src/pot3d.F90:6187-6233) within 1e-8 relative to their largest magnitude,
+magnetic energy (:6235-6321) within 1e-7 relative, recomputed FP64
+||b - A x||2/||b||2 <= 1e-9 where the stop is on it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PHI_TOL = 1e-8
+B_TOL = 1e-8
+E_TOL = 1e-7
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _maxrel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _check_fields(s, phi, phi_ref, B_ref, w_ref, remove_mean=False, label=""):
+    """phi, B and energy of the GPU solver's current phi against a reference."""
+    if remove_mean:
+        d = _maxrel(phi - phi.mean(), phi_ref - phi_ref.mean())
+    else:
+        d = _maxrel(phi, phi_ref)
+    B = s.getb(phi)
+    dB = max(_maxrel(g, r) for g, r in zip(B, B_ref))
+    w = s.magnetic_energy(phi)
+    dE = float(np.max(np.abs(w - w_ref) / np.abs(w_ref)))
+    say(f"    {label}: phi {d:.2e}  B {dB:.2e}  energy {dE:.2e}")
+    assert d <= PHI_TOL and dB <= B_TOL and dE <= E_TOL, (d, dB, dE)
+    return d, dB, dE
+
+
+def _set(s, **opts):
+    from pot3d_amd import _lib
+    for k, v in opts.items():
+        s.set_option(getattr(_lib, "OPT_" + k.upper()), v)
+
+
+def test_cfg1_pfss_fp64_diag_vs_oracle(oracle_mod):
+    """cfg1: FP64 PCG, diagonal preconditioner, the reference's epsn stop."""
+    from pot3d_amd import solver, workloads
+    w = workloads.config(1)
+    say(f"[gpu] cfg1 {w.name} N={w.N}")
+    o = oracle_mod.Oracle(w.mesh, w.pm_r1)
+    b = o.rhs(w.br0)
+    ref = o.cgsolve(b, ifprec=1, epscg=1e-9)
+    phir = o.fill_phi(ref["x"], w.br0)
+    with solver.Pot3dSolver(w.mesh, w.pm_r1, 1) as s:
+        s.set_br0(w.br0)
+        x = np.zeros(w.N)
+        ierr, ncg, epsn = s.solve(x, None, 1e-9, ncgmax=20000)
+        say(f"    ncg {ncg} (oracle {ref['ncg']}), epsn {epsn:.3e}")
+        assert ierr == 0 and epsn < 1e-9 and abs(ncg - ref["ncg"]) <= 2
+        assert _maxrel(x, ref["x"]) <= PHI_TOL
+        phi = s.fill_phi()
+        _check_fields(s, phi, phir, o.getb(phir), o.energy(phir), label="fp64 diag")
+        assert s.stats()["true_relres"] < 1e-6
+
+
+@pytest.mark.parametrize("option", ["ss", "potential"])
+@pytest.mark.parametrize("mixed", [1, 2])
+def test_cfg3_mixed_ilu_vs_oracle(oracle_mod, option, mixed):
+    """cfg3: FP32 inner ILU0-PCG (inner tolerance 1e-5 in mode 1) inside
+    FP64 defect correction to ||b - A x||2/||b||2 <= 1e-9, on the cfg1 grid,
+    with phi=0 at r1 ('ss') and the Neumann variant ('potential')."""
+    from pot3d_amd import solver, workloads
+    w = workloads.pfss(51, 91, 181, seed=0, option=option, ifprec=2)
+    say(f"[gpu] cfg3 {w.name} option={option} mixed={mixed}")
+    o = oracle_mod.Oracle(w.mesh, w.pm_r1)
+    b = o.rhs(w.br0)
+    ref = o.cgsolve(b, ifprec=3, epscg=1e-12)
+    assert ref["ierr"] == 0
+    phir = o.fill_phi(ref["x"], w.br0)
+    Br = o.getb(phir)
+    wr = o.energy(phir)
+    singular = option == "potential"
+    with solver.Pot3dSolver(w.mesh, w.pm_r1, 2) as s:
+        s.set_br0(w.br0)
+        _set(s, stop=1, mixed=mixed, inner_tol=1e-5)
+        x = np.zeros(w.N)
+        ierr, ncg, epsn = s.solve(x, None, 1e-9, ncgmax=20000)
+        st = s.stats()
+        say(f"    ncg {ncg} outer {st['n_outer']} true relres {st['true_relres']:.3e} "
+            f"(oracle FP64 ncg {ref['ncg']} to epsn 1e-12)")
+        assert ierr == 0 and st["true_relres"] <= 1e-9 and st["n_outer"] >= 1
+        # the oracle's operator agrees on the residual
+        rr = np.linalg.norm(b - o.ax(x)) / np.linalg.norm(b)
+        assert rr <= 1.05e-9, rr
+        phi = s.fill_phi()
+        _check_fields(s, phi, phir, Br, wr, remove_mean=singular,
+                      label=f"mixed {mixed} {option}")
+        # the FP64 GPU solve of the same problem, for the iteration count
+        _set(s, mixed=0)
+        x64 = np.zeros(w.N)
+        assert s.solve(x64, None, 1e-9, ncgmax=20000)[0] == 0
+        n64 = s.stats()["ncg"]
+        say(f"    fp64 GPU ncg {n64}")
+        assert ncg <= 1.3 * n64 + 20
+
+
+def _fp64_then_mixed(w, blocks, label, oracle_check=None):
+    from pot3d_amd import solver
+    with solver.Pot3dSolver(w.mesh, w.pm_r1, 2) as s:
+        s.set_br0(w.br0)
+        _set(s, stop=1)
+        x64 = np.zeros(w.N)
+        ierr, ncg64, _ = s.solve(x64, None, 1e-9, ncgmax=20000)
+        st = s.stats()
+        say(f"    {label} fp64 ILU0: ncg {ncg64} true relres {st['true_relres']:.3e} "
+            f"t {st['t_solve']:.2f}s")
+        assert ierr == 0 and st["true_relres"] <= 1e-9
+        if oracle_check:
+            oracle_check(s, x64)
+        phi64 = s.fill_phi()
+        B64 = s.getb(phi64)
+        w64 = s.magnetic_energy(phi64)
+        _set(s, ilu_blocks_r=blocks[0], ilu_blocks_t=blocks[1], ilu_blocks_p=blocks[2],
+             tile_k=16, tile_stage=1, mixed=2)
+        x = np.zeros(w.N)
+        ierr, ncg, _ = s.solve(x, None, 1e-9, ncgmax=20000)
+        st = s.stats()
+        say(f"    {label} mixed block-Jacobi {blocks}: ncg {ncg} outer {st['n_outer']} "
+            f"true relres {st['true_relres']:.3e} t {st['t_solve']:.2f}s")
+        assert ierr == 0 and st["true_relres"] <= 1e-9
+        phi = s.fill_phi()
+        del x
+        _check_fields(s, phi, phi64, B64, w64, label=f"{label} mixed vs fp64")
+
+
+def test_cfg2_fp64_residual_and_mixed(oracle_mod):
+    """cfg2 151x301x601: the FP64 ILU0 solve's residual recomputed by the
+    oracle's operator; mixed block-Jacobi solve against it."""
+    from pot3d_amd import workloads
+    w = workloads.config(2)
+    say(f"[gpu] cfg2 {w.name} N={w.N}")
+
+    def oracle_residual(s, x):
+        o = oracle_mod.Oracle(w.mesh, w.pm_r1)
+        b = o.rhs(w.br0)
+        rr = np.linalg.norm(b - o.ax(x)) / np.linalg.norm(b)
+        say(f"    oracle ||b - A x||/||b|| = {rr:.3e}")
+        assert rr <= 1.05e-9
+
+    _fp64_then_mixed(w, (8, 16, 16), "cfg2", oracle_residual)
+
+
+@pytest.mark.timeout(600)
+def test_cfg5_open_mixed_vs_fp64():
+    """cfg5 301x601x1201 'open': mixed block-Jacobi solve against the FP64
+    ILU0 solve (phi, B, energy; recomputed residuals of both)."""
+    from pot3d_amd import workloads
+    w = workloads.config(5)
+    say(f"[gpu] cfg5 {w.name} N={w.N}")
+    _fp64_then_mixed(w, (8, 16, 16), "cfg5")

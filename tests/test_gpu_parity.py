@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import random_br0, small_mesh
+from conftest import ROOT, random_br0, small_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -110,28 +110,22 @@ def test_ilu_pivots_and_apply(oracle_mod, solver_mod, name):
 
 @pytest.mark.parametrize("dims", [(9, 8, 12), (13, 11, 18), (21, 31, 48), (40, 19, 30),
                                   (33, 45, 70)])
-def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
-    """The three triangular-solve schedules (tile diagonals, one launch per
-    hyperplane, one persistent launch per sweep) do the same arithmetic per
-    point: results must be bit-identical, on every repeat (the persistent
-    sweep re-arms its tickets and progress counters itself)."""
-    say(f"[gpu] ilu dataflow vs hyperplanes dims={dims}")
+def test_ilu_tile_sweeps_match_hyperplanes(solver_mod, dims):
+    """The triangular-solve schedules (tile diagonals of every tile shape,
+    one launch per hyperplane) do the same arithmetic per point: results
+    must be bit-identical, on every repeat."""
+    say(f"[gpu] ilu tile sweeps vs hyperplanes dims={dims}")
     from pot3d_amd import _lib
     mesh = small_mesh(*dims)
     with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
         rng = np.random.default_rng(8)
         for rep in range(3):
             x = rng.standard_normal(s.N)
-            s.set_option(_lib.OPT_SWEEP, 2)
-            z_df = s.prec_inv(x)
-            z_df2 = s.prec_inv(x)
             s.set_option(_lib.OPT_SWEEP, 1)
             z_lv = s.prec_inv(x)
             s.set_option(_lib.OPT_SWEEP, 0)
             z_tl = s.prec_inv(x)
             z_tl2 = s.prec_inv(x)
-            assert np.array_equal(z_df, z_lv), (rep, np.max(np.abs(z_df - z_lv)))
-            assert np.array_equal(z_df, z_df2)
             assert np.array_equal(z_tl, z_lv), (rep, np.max(np.abs(z_tl - z_lv)))
             assert np.array_equal(z_tl, z_tl2)
             # other tile shapes: waves per tile x planes per tile, and
@@ -150,9 +144,28 @@ def test_ilu_dataflow_sweep_matches_hyperplanes(solver_mod, dims):
             s.set_option(_lib.OPT_TILE_STAGE, 0)
 
 
-@pytest.mark.skip(reason="the persistent dataflow sweep (P3D_OPT_SWEEP=2) inside the CG "
-                         "iteration loop did not terminate on the MI355X (three runs); the "
-                         "solver rejects that schedule until the cause is shown")
+_DATAFLOW_REMOVED = ("the persistent dataflow ILU sweep (P3D_OPT_SWEEP=2: one launch per "
+                     "sweep, tiles waiting on each other's progress counters) did not "
+                     "terminate inside the CG iteration loop on the MI355X (three runs); "
+                     "its cause was not shown, and the schedule has been removed from the "
+                     "library")
+
+
+@pytest.mark.skip(reason=_DATAFLOW_REMOVED)
+def test_ilu_dataflow_sweep_single_apply(solver_mod):
+    """Single applications of the persistent dataflow sweep against the
+    hyperplane schedule (bit for bit)."""
+    from pot3d_amd import _lib
+    mesh = small_mesh(21, 31, 48)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
+        x = np.random.default_rng(8).standard_normal(s.N)
+        s.set_option(_lib.OPT_SWEEP, 2)
+        z_df = s.prec_inv(x)
+        s.set_option(_lib.OPT_SWEEP, 1)
+        assert np.array_equal(z_df, s.prec_inv(x))
+
+
+@pytest.mark.skip(reason=_DATAFLOW_REMOVED)
 def test_ilu_solve_same_with_both_sweeps(solver_mod):
     say("[gpu] ilu solve dataflow vs hyperplanes")
     from pot3d_amd import _lib
@@ -216,15 +229,33 @@ def test_ilu_solve_tiles_vs_hyperplanes(solver_mod):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_dataflow_sweep_rejected_in_solve(solver_mod):
+def test_dataflow_sweep_option_rejected(solver_mod):
+    """The removed persistent schedule is refused at the option."""
     from pot3d_amd import _lib
     mesh = small_mesh()
     with solver_mod.Pot3dSolver(mesh, -1.0, 2) as s:
-        s.set_br0(random_br0(mesh, 1))
-        s.set_option(_lib.OPT_SWEEP, 2)
         with pytest.raises(_lib.Pot3dError) as e:
-            s.solve(np.zeros(s.N), None, 1e-9, ncgmax=10)
+            s.set_option(_lib.OPT_SWEEP, 2)
         assert e.value.code == _lib.P3D_EINVAL
+
+
+def test_ilu_pivot_guard(solver_mod):
+    """A non-finite mesh entry makes the D-ILU pivots non-finite: the
+    factorisation refuses them (P3D_ENONFINITE) instead of iterating on
+    inf/NaN (the reference's host ilu0 prints a zero pivot and goes on,
+    src/pot3d.F90:4695-4697)."""
+    say("[gpu] ILU pivot guard")
+    from pot3d_amd import _lib
+    mesh = small_mesh(13, 11, 18)
+    mesh.dph = mesh.dph.copy()
+    mesh.dph[5] = np.nan
+    with pytest.raises(_lib.Pot3dError) as e:
+        solver_mod.Pot3dSolver(mesh, -1.0, 2)
+    assert e.value.code == _lib.P3D_ENONFINITE
+    assert "pivot" in str(e.value)
+    # the same mesh with the diagonal preconditioner is created (no pivots)
+    with solver_mod.Pot3dSolver(mesh, -1.0, 1):
+        pass
 
 
 @pytest.mark.parametrize("ifprec,oprec", [(1, 1), (2, 3), (0, 0)])
@@ -383,6 +414,121 @@ def test_legacy_lu_abi(oracle_mod, solver_mod):
     L.external_lu_solver(y.ctypes.data)
     assert np.max(np.abs(y - ref)) <= 1e-4 * np.max(np.abs(ref))
     L.unload_external_lu_solver()
+
+
+def test_legacy_lu_abi_device_pointers(oracle_mod, solver_mod):
+    """The -DCUSPARSE calling convention (src/pot3d.F90:4675-4683,
+    use_device_addr): CSR arrays and x are device addresses.  The LU factors
+    are written back over the device CSR values, x is solved in place on the
+    device, and IA/JA are left as they were; repeated applies give the same
+    bits as the host-array convention."""
+    say("[gpu] legacy ABI, device pointers")
+    from pot3d_amd import _hip, _lib
+    H = _hip.Hip()
+    L = _lib.load()
+    mesh = small_mesh(17, 14, 26, drratio=2.0)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    IA, JA, Av, dptr = o.csr()
+    _, _, LUref, _, _ = o.ilu0()
+    nnz = IA[-1] - 1
+    x = np.random.default_rng(5).standard_normal(o.N)
+    # host convention first, for the bit-for-bit comparison
+    lu_h = Av.copy()
+    L.load_external_lu_solver(lu_h.ctypes.data, IA.ctypes.data, JA.ctypes.data, o.N, len(Av))
+    y_h = x.copy()
+    L.external_lu_solver(y_h.ctypes.data)
+    L.unload_external_lu_solver()
+    bufs = [H.to_device(a) for a in (Av, IA, JA, x)]
+    d_av, d_ia, d_ja, d_x = bufs
+    try:
+        L.load_external_lu_solver(d_av, d_ia, d_ja, o.N, len(Av))
+        lu = np.empty_like(Av)
+        H.d2h(lu, d_av)
+        assert np.max(np.abs(lu[:nnz] - LUref[:nnz]) / np.abs(LUref[:nnz])) < 1e-4
+        assert np.array_equal(lu[:nnz], lu_h[:nnz])
+        ia2, ja2 = np.empty_like(IA), np.empty_like(JA)
+        H.d2h(ia2, d_ia)
+        H.d2h(ja2, d_ja)
+        assert np.array_equal(ia2, IA) and np.array_equal(ja2, JA)
+        ref = o.lusol(x.astype(np.float32)).astype(np.float64)
+        y = np.empty_like(x)
+        for _ in range(2):
+            H.h2d(d_x, x)
+            L.external_lu_solver(d_x)
+            H.d2h(y, d_x)
+            assert np.max(np.abs(y - ref)) <= 1e-4 * np.max(np.abs(ref))
+            assert np.array_equal(y, y_h)
+        L.unload_external_lu_solver()
+    finally:
+        for d in bufs:
+            H.free(d)
+
+
+def test_legacy_lu_abi_zero_pivot_exits(oracle_mod, tmp_path):
+    """A zero ILU0 pivot ends the program with a message and exit status 1,
+    the shim's error convention (the reference's host ilu0 reports zero
+    pivots, src/pot3d.F90:4695-4697).  Run in a child process."""
+    say("[gpu] legacy ABI zero pivot")
+    import subprocess
+    mesh = small_mesh(13, 11, 18)
+    o = oracle_mod.Oracle(mesh, -1.0)
+    IA, JA, Av, dptr = o.csr()
+    Av[IA[0] - 1:IA[1] - 1] = 0.0   # row 1: zero diagonal and couplings
+    np.savez(tmp_path / "csr.npz", IA=IA, JA=JA, Av=Av)
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "from pot3d_amd import _lib\n"
+        f"z = np.load({repr(str(tmp_path / 'csr.npz'))})\n"
+        "IA, JA, Av = z['IA'], z['JA'], z['Av'].copy()\n"
+        "L = _lib.load()\n"
+        "L.load_external_lu_solver(Av.ctypes.data, IA.ctypes.data, JA.ctypes.data, "
+        "len(IA) - 1, len(Av))\n"
+        "print('returned', flush=True)\n")
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=120)
+    say(res.stdout.strip(), res.stderr.strip()[-300:])
+    assert res.returncode == 1
+    assert "zero or non-finite pivot" in res.stderr and "row 1" in res.stderr
+    assert "returned" not in res.stdout
+
+
+def test_neumann_r1_boundary_values(oracle_mod, solver_mod):
+    """The validation run's Neumann value at r1 (br1, src/pot3d.F90:5420-
+    5423; rhs at :3941-3950): rhs, phi with boundary values, energy and B of
+    a solve against the oracle.  With Neumann ghosts at both r ends the
+    system is singular (constants); br0 and br1 are both flux balanced, so it
+    is consistent, and phi is compared after removing its mean."""
+    say("[gpu] br1 Neumann r1 values")
+    mesh = small_mesh(21, 31, 48, drratio=2.5)
+    o = oracle_mod.Oracle(mesh, 1.0)
+    br0 = random_br0(mesh, 21)
+    br1 = 0.2 * random_br0(mesh, 22)
+    b = o.rhs(br0, br1)
+    bz = o.rhs(br0)
+    assert np.max(np.abs(b - bz)) > 1e-3 * np.max(np.abs(b))  # br1 enters
+    ref = o.cgsolve(b, ifprec=3, epscg=1e-12)
+    phir = o.fill_phi(ref["x"], br0, br1)
+    with solver_mod.Pot3dSolver(mesh, 1.0, 2) as s:
+        s.set_br0(br0)
+        s.set_br1(br1)
+        assert np.max(np.abs(s.rhs() - b)) <= 1e-14 * np.max(np.abs(b))
+        xr = np.random.default_rng(3).standard_normal(o.N)
+        pf = s.fill_phi(xr)
+        pref = o.fill_phi(xr, br0, br1)
+        assert np.max(np.abs(pf - pref)) <= 1e-14 * np.max(np.abs(pref))
+        x = np.zeros(o.N)
+        ierr, ncg, epsn = s.solve(x, None, 1e-12, ncgmax=5000)
+        say(f"    ncg {ncg} (oracle {ref['ncg']})")
+        assert ierr == 0 and abs(ncg - ref["ncg"]) <= 2
+        phi = s.fill_phi()
+        dm = (phi - phi.mean()) - (phir - phir.mean())
+        assert np.max(np.abs(dm)) <= 1e-8 * np.max(np.abs(phir - phir.mean()))
+        for g_, r_ in zip(s.getb(), o.getb(phir)):
+            assert np.max(np.abs(g_ - r_)) <= 1e-8 * np.max(np.abs(r_))
+        assert np.allclose(s.magnetic_energy(), o.energy(phir), rtol=1e-7, atol=0)
+        s.set_br1(None)
+        assert np.max(np.abs(s.rhs() - bz)) <= 1e-14 * np.max(np.abs(bz))
 
 
 def _blk_extents(n, nb, q):
