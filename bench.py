@@ -1,32 +1,40 @@
 """Benchmark: PCG time-to-solution of the POT3D Laplace solve on one MI355X.
 
-Workload: 151x301x601 PFSS (26.7 M unknowns), r in [1,2.5] with a geometric
-radial mesh, uniform theta/phi, seeded synthetic Br (8 bipoles + l<=15
-background), phi=0 at r=2.5, ILU0 preconditioner, stopping at
-||b - Ax||2/||b||2 <= 1e-9.  A step is one complete solve from the zero
-guess with the vectors already resident in HBM.
+Workload (default, --config 5): 301x601x1201 PFCS-style 'open' problem
+(214.7 M unknowns), r in [2.5, 10] with a geometric radial mesh, uniform
+theta/phi, seeded synthetic Br (16 bipoles + l<=15 background) with the
+|Br| treatment and phi=0 at r1, ILU0 preconditioner, stopping at
+||b - Ax||2/||b||2 <= 1e-9.  --config 2 runs the 151x301x601 PFSS problem
+(r in [1, 2.5], 8 bipoles, phi=0 at 2.5).  A step is one complete solve
+with the vectors already resident in HBM.
 
 Reported:
-  value        time-to-solution of the fastest path, seconds: FP32 inner
-               PCG with reliable updates in FP64 defect correction, the
-               block-Jacobi ILU0 (the reference's per-rank ILU0 on a grid cut
-               into 8x16x16 blocks) and the learned warm start (network trained
-               on 51x91x181 PCG solutions, prolonged to the fine grid); the
-               timed region includes the network's prediction
-  fp64_plain   the same solve with the plain FP64 single-rank ILU0 PCG (the
-               drop-in baseline), seconds, and the speed-up; also the FP64
-               block-Jacobi and the mixed path without the network
-  roofline     the reported path's dominant kernel (the FP32 block-Jacobi
-               ILU0 apply, forward and backward tile sweeps): achieved HBM
-               bandwidth (algorithmic bytes / HIP-event time on the solver's
-               stream) vs 8 TB/s, and its measured HBM traffic
-  cpu_baseline the CPU oracle's ILU0 PCG (the reference's CPU algorithm:
-               Saad ilu0 + lusol) timed on a bounded sample on the host
+  value        time-to-solution of the reported path, seconds: FP32 inner
+               PCG with the FP64 solution (reliable updates), the
+               block-Jacobi ILU0 (the reference's per-rank ILU0 on a grid
+               cut into 8x16x16 blocks) and the learned warm start (network
+               trained on 51x91x181 PCG solutions of the same problem family,
+               weights committed under pot3d_amd/weights/, prolonged to the
+               fine grid); the network's prediction is inside the timed region
+  fp64_plain   the plain FP64 single-rank ILU0 PCG (the drop-in baseline),
+               and the speed-up over it; fp64_blockjacobi the FP64 solve with
+               the value path's preconditioner (the same-preconditioner
+               ratio); mixed the value path without the network
+  errors       potential, B field (getb) and magnetic energy of each fast path
+               against the FP64 solve, asserted against 1e-8 / 1e-8 / 1e-7
+  roofline     the value path's dominant kernel (the FP32 block-Jacobi ILU0
+               apply): algorithmic bytes / HIP-event time on the solver's
+               stream vs 8 TB/s, and its measured HBM traffic; also the whole
+               iteration's algorithmic bytes per iteration time
+  cpu_baseline the reference's CPU ifprec=2 algorithm (per-rank diacsr +
+               Saad ilu0 + lusol, block-Jacobi over P MPI ranks, P = the host
+               cores this job may use) restated in the oracle: a bounded
+               sample at this workload, plus a full solve to convergence of
+               the 51x91x181 problem of the same family
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -38,37 +46,72 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PHI_TOL, B_TOL, E_TOL = 1e-8, 1e-8, 1e-7
 
 from pot3d_amd._hip import Hip  # noqa: E402
 
+# config number -> (problem family, grid, label)
+CONFIGS = {
+    5: ("open", (301, 601, 1201), "PFCS-style open"),
+    2: ("pfss", (151, 301, 601), "PFSS"),
+}
 
-def cpu_baseline(workload, gpu_iters: int, sample_iters: int, threads: int):
-    """The oracle's ILU0 PCG (ifprec=2: diacsr + Saad ilu0 + lusol, the
-    reference's CPU path) on the same problem: set-up plus `sample_iters`
-    iterations timed, extrapolated to the GPU's iteration count."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+
+def host_cores() -> int:
+    """Host cores this job may use: OMP_NUM_THREADS when the environment
+    sets it (the GPU host grants a share of its cores), else all."""
+    v = os.environ.get("OMP_NUM_THREADS")
+    try:
+        return max(1, int(v)) if v else (os.cpu_count() or 1)
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(workload, family: str, cores: int, sample_iters: int, n_iter_est: int):
+    """The reference's CPU ifprec=2 solve on `cores` MPI ranks (the oracle's
+    restatement, one OpenMP thread per rank): set-up and `sample_iters`
+    iterations timed at this workload and scaled to n_iter_est iterations;
+    and the 51x91x181 problem of the same family solved to convergence."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    o = oracle.Oracle(workload.mesh, workload.pm_r1)
+    from p3d_driver import workloads
+    m = workload.mesh
+    topo = oracle.rank_topology(m.nr, m.nt, m.np, cores)
+    o = oracle.Oracle(m, workload.pm_r1)
     t0 = time.perf_counter()
     b = o.rhs(workload.br0)
     o.coefs()
     t_coef = time.perf_counter() - t0
+    res = o.cgsolve_ranks(b, topo, epscg=1e-9, stop_mode=1, ncgmax=sample_iters)
+    per_iter = res["t_iter"] / max(res["ncg"], 1)
+    t_ilu = res["t_setup"]
+    est = t_coef + t_ilu + per_iter * n_iter_est
+    del o, b, res
+    # the coarse problem of the same family, to convergence
+    wc = workloads.domain_workload(family, workloads.COARSE_GRID, 0)
+    mc = wc.mesh
+    topc = oracle.rank_topology(mc.nr, mc.nt, mc.np, cores)
+    oc = oracle.Oracle(mc, wc.pm_r1)
     t0 = time.perf_counter()
-    o.cgsolve(b, ifprec=2, epscg=1e-30, ncgmax=0)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    o.cgsolve(b, ifprec=2, epscg=1e-30, ncgmax=sample_iters)
-    t_run = time.perf_counter() - t0 - t_setup
-    per_iter = t_run / sample_iters
-    est = t_coef + t_setup + per_iter * gpu_iters
-    return {"value": round(est, 3), "unit": "s (estimated time-to-solution)", "cores": threads,
-            "kind": "port",
-            "sample": (f"oracle ILU0 PCG on the same problem ({workload.name}): matrix+ILU0 set-up "
-                       f"({t_coef + t_setup:.2f} s) and {sample_iters} iterations "
-                       f"({per_iter:.3f} s/iteration, {threads} OpenMP threads for the "
-                       f"operator, sequential trisolves), extrapolated to the GPU's "
-                       f"{gpu_iters} iterations")}
+    bc = oc.rhs(wc.br0)
+    oc.coefs()
+    t_cc = time.perf_counter() - t0
+    rc = oc.cgsolve_ranks(bc, topc, epscg=1e-9, stop_mode=1)
+    full = {"workload": wc.name, "ranks": list(topc), "ierr": rc["ierr"], "ncg": rc["ncg"],
+            "t_total_s": round(t_cc + rc["t_setup"] + rc["t_iter"], 3),
+            "t_setup_s": round(t_cc + rc["t_setup"], 3), "t_iter_s": round(rc["t_iter"], 3)}
+    return {"value": round(est, 2), "unit": "s (estimated time-to-solution)", "cores": cores,
+            "kind": "port", "ranks": list(topo),
+            "sample": (f"the oracle's restatement of the reference's CPU ifprec=2 solve "
+                       f"(per-rank diacsr + Saad ilu0 + lusol, block-Jacobi over "
+                       f"{topo[0]}x{topo[1]}x{topo[2]} ranks, one thread each, stored FP64 "
+                       f"operator) on this workload: coefficients {t_coef:.1f} s, ILU0 set-up "
+                       f"{t_ilu:.1f} s and "
+                       f"{sample_iters} iterations at {per_iter:.3f} s each, scaled to "
+                       f"{n_iter_est} iterations (the single-rank ILU0's count: a lower bound, "
+                       f"the block-Jacobi cut only adds iterations)"),
+            "per_iter_s": round(per_iter, 4), "t_setup_s": round(t_coef + t_ilu, 2),
+            "full_solve_coarse": full}
 
 
 # (name, time_kernel id, algorithmic bytes per unknown, padded-array bytes per point)
@@ -87,8 +130,7 @@ KERNELS = [
 def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
     """Average HIP-event time per launch on the solver's stream, algorithmic
     bytes per launch and the achieved bandwidth of each kernel.  graph=False
-    leaves out the graph-replayed iteration (profiler runs: rocprofv3's
-    kernel trace of graph launches ended in a host crash on this image)."""
+    leaves out the graph-replayed iteration."""
     table = {}
     for name, kid, per_n, per_pad in KERNELS:
         if kid == 3 and not graph:
@@ -100,9 +142,9 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
     return {"table": table}
 
 
-# Kernels of the reported (fastest) path: FP32 recurrence with the
-# block-Jacobi ILU0 on single-wave tiles.  (name, time_kernel id, algorithmic
-# bytes per unknown, padded-array bytes per point)
+# Kernels of the value path's iteration: FP32 recurrence with the
+# block-Jacobi ILU0 on single-wave tiles.  (name, time_kernel id,
+# algorithmic bytes per unknown, padded-array bytes per point)
 VALUE_KERNELS = [
     ("ilu_apply_fp32_blockjacobi (k_tile_fwd_lds+k_tile_bwd_lds<float>)", 7, 24, 0),
     ("stencil_fp32 (k_stencil_march<float>)", 1, 4, 4),
@@ -128,9 +170,9 @@ def set_fast_ilu(s, args):
 
 
 def measure_value_kernels(s, N: int, padded: int, reps: int = 5):
-    """The reported path's kernels (set_fast_ilu applied), timed as
-    measure_kernels; the ILU apply is the dominant one (about 40% of an
-    iteration), so it is the roofline kernel."""
+    """The value path's kernels (set_fast_ilu applied), timed as
+    measure_kernels; the ILU apply is the dominant one, so it is the
+    roofline kernel.  Also the algorithmic bytes of one iteration."""
     table = {}
     for name, kid, per_n, per_pad in VALUE_KERNELS:
         ms = s.time_kernel(kid, reps)
@@ -138,61 +180,51 @@ def measure_value_kernels(s, N: int, padded: int, reps: int = 5):
         table[name] = {"ms": round(ms, 5), "bytes": nbytes,
                        "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
     name = VALUE_KERNELS[0][0]
-    return {"table": table, "dominant": dict(name=name, **table[name])}
+    iter_bytes = sum(v["bytes"] for v in table.values())
+    iter_ms = sum(v["ms"] for v in table.values())
+    return {"table": table, "dominant": dict(name=name, **table[name]),
+            "iter_bytes": iter_bytes, "kernels_ms": iter_ms}
 
 
-def _coarse_br(args_tuple):
-    """br0 of the seeded synthetic map on the coarse mesh (worker process)."""
-    mesh, seed, nbip, option = args_tuple
-    from pot3d_amd import inputs
-    return inputs.synthetic_br0(mesh, seed, n_bipoles=nbip, option=option)
-
-
-def make_brs(mesh, seeds, workers: int, nbip: int = 8, option: str = "ss"):
-    """Boundary maps for the training set, generated on the host before the
-    GPU is touched (a fork-based pool; workers=1 runs inline)."""
-    jobs = [(mesh, sd, nbip, option) for sd in seeds]
-    if workers <= 1 or len(seeds) < 8:
-        return [_coarse_br(j) for j in jobs]
-    import multiprocessing as mp
-    with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(_coarse_br, jobs, chunksize=8)
-
-
-def measured_traffic(workload: str, kernel: str):
+def measured_traffic(grid: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) for this
-    workload, or None when there is none."""
+    grid, or None when there is none."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get(workload, {}).get(kernel)
+    k = t.get(grid, {}).get(kernel)
     return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
+
+
+def default_weights(family: str) -> str:
+    return os.path.join(ROOT, "pot3d_amd", "weights", f"warmstart_{family}_51x91x181.npz")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3, help="timed solves of the value path")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--grid", type=str, default="151x301x601")
+    ap.add_argument("--ref-runs", type=int, default=1,
+                    help="timed solves of each comparison path (FP64 plain, FP64 "
+                         "block-Jacobi, mixed without the network)")
+    ap.add_argument("--config", type=int, default=5, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=3,
+                    help="iterations of the CPU sample at the full workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the kernels (for profiler runs)")
+    ap.add_argument("--profile-solve", action="store_true",
+                    help="one value-path solve without graph capture (for kernel traces)")
     ap.add_argument("--mixed-mode", type=int, default=2,
                     help="1 restarted defect correction, 2 reliable updates")
-    ap.add_argument("--ws-pairs", type=int, default=512,
-                    help="training pairs of the warm-start network (0: no network)")
-    ap.add_argument("--ws-epochs", type=int, default=3)
-    ap.add_argument("--ws-lr", type=float, default=1e-3)
-    ap.add_argument("--ws-grid", type=str, default="51x91x181")
-    ap.add_argument("--large", action="store_true",
-                    help="the 301x601x1201 PFCS-style problem: 'open' at r in [2.5, 10], "
-                         "16 bipoles (one step, no CPU baseline by default)")
+    ap.add_argument("--weights", type=str, default="",
+                    help="warm-start weights (.npz from pot3d_amd.train_warmstart); default "
+                         "the committed ones of the problem family; 'none' skips the network")
     ap.add_argument("--blocks", type=str, default="8,16,16",
                     help="block-Jacobi ILU0 blocks (r,theta,phi) of the fast paths; "
                          "1,1,1 = single-rank ILU0")
@@ -201,8 +233,8 @@ def main():
     ap.add_argument("--tile-stage", type=int, default=1,
                     help="ILU tiles of the fast paths staged by the marching wave alone (1) "
                          "or by planes/8 waves (0)")
-    ap.add_argument("--no-levels", action="store_true",
-                    help="skip the FP64 run with hyperplane-level trisolves")
+    ap.add_argument("--levels", action="store_true",
+                    help="also time the FP64 solve with hyperplane-level trisolves")
     ap.add_argument("--guess-check", type=int, default=2,
                     help="1 residual rule, 2 A-norm rule for accepting the learned guess")
     args = ap.parse_args()
@@ -211,31 +243,27 @@ def main():
               f"(--gpus {args.gpus} requested); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from pot3d_amd import workloads
-    dom = dict(option="ss", r0=1.0, r1=2.5, n_bipoles=8)
-    if args.large:
-        args.grid = "301x601x1201"
-        dom = dict(option="open", r0=2.5, r1=10.0, n_bipoles=16)
-    nr, nt, np_ = (int(v) for v in args.grid.split("x"))
+    from p3d_driver import workloads
+    family, grid, label = CONFIGS[args.config]
+    gname = "x".join(map(str, grid))
     t0 = time.perf_counter()
-    w = workloads.pfss(nr, nt, np_, seed=0, ifprec=2, name=f"pfss_{args.grid}", **dom)
+    w = workloads.config(args.config)
+    dom = workloads.DOMAINS[family]
     print(f"[bench] workload {w.name} N={w.N} built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    ws = None
-    if args.ws_pairs > 0 and not args.kernels_only:
-        cnr, cnt, cnp = (int(v) for v in args.ws_grid.split("x"))
-        wc = workloads.pfss(cnr, cnt, cnp, seed=0, ifprec=2, name=f"pfss_{args.ws_grid}", **dom)
-        t0 = time.perf_counter()
-        seeds = list(range(1000, 1000 + args.ws_pairs))
-        brs = make_brs(wc.mesh, seeds, min(16, os.cpu_count() or 1), dom["n_bipoles"],
-                       dom["option"])
-        ws = dict(wc=wc, brs=brs, t_maps=time.perf_counter() - t0)
-        print(f"[bench] {len(brs)} training maps on {args.ws_grid} in {ws['t_maps']:.1f}s",
+    wpath = args.weights or default_weights(family)
+    use_net = wpath != "none" and not args.kernels_only
+    if use_net and not os.path.exists(wpath):
+        print(f"[bench] no warm-start weights at {wpath}: value path without the network",
               file=sys.stderr, flush=True)
+        use_net = False
+    wc = workloads.domain_workload(family, workloads.COARSE_GRID, 0) if use_net else None
+    cores = host_cores()
 
     from pot3d_amd import _lib, solver
     hip = Hip()
     N = w.N
+    nr, nt, np_ = grid
     s = solver.Pot3dSolver(w.mesh, w.pm_r1, 2)
     s.set_br0(w.br0)
     s.set_option(_lib.OPT_STOP, 1)
@@ -270,11 +298,11 @@ def main():
         assert ierr == 0, f"solve did not converge ({st})"
         return dt, st
 
-    def timed(label, mixed, guess=None):
-        for _ in range(args.warmup):
+    def timed(label, mixed, guess=None, warmup=0, steps=1):
+        for _ in range(warmup):
             run(mixed, guess)
         ts = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             dt, st = run(mixed, guess)
             ts.append(dt)
             print(f"[bench] {label}: {dt:.3f}s ncg={st['ncg']} outer={st['n_outer']} "
@@ -284,14 +312,24 @@ def main():
         hip.d2h(x, d_x)
         return ts, st, x
 
+    if args.profile_solve:
+        # one value-path solve with every kernel launched on its own
+        set_fast_ilu(s, args)
+        s.set_option(_lib.OPT_GRAPH, 0)
+        ts, st, _ = timed("mixed block-Jacobi ILU0 PCG, no graph", args.mixed_mode)
+        print(json.dumps({"workload": w.name, "profile_solve_s": round(ts[0], 4),
+                          "ncg": st["ncg"]}))
+        s.close()
+        return
+
     # FP64 plain ILU0 PCG (drop-in baseline: the single-rank ILU0)
-    t64, st64, x64 = timed("fp64 ILU0 PCG", 0)
-    w64 = s.magnetic_energy(s.fill_phi(x64))
-    # the same solve with the triangular solves level-scheduled by i+j+k
-    # hyperplanes (one launch per hyperplane): the structure of the
-    # reference's GPU ILU0 path; one untimed-warm-up-free run
+    t64, st64, x64 = timed("fp64 ILU0 PCG", 0, steps=args.ref_runs)
+    phi64 = s.fill_phi(x64)
+    B64 = s.getb(phi64)
+    w64 = s.magnetic_energy(phi64)
+    del phi64
     tlv = None
-    if not args.no_levels:
+    if args.levels:
         s.set_option(_lib.OPT_SWEEP, 1)
         dt, stlv = run(0)
         tlv = (dt, stlv["ncg"])
@@ -301,39 +339,34 @@ def main():
     kern = measure_kernels(s, N, nr * nt * np_)
     blocks = set_fast_ilu(s, args)
     vkern = measure_value_kernels(s, N, nr * nt * np_)
-    ilu_bj = {"fp64_ms": round(s.time_kernel(2, 5), 5),
-              "fp32_ms": vkern["table"][VALUE_KERNELS[0][0]]["ms"]}
-    tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0)
-    # mixed precision from the zero guess
-    tmx, stmx, xmx = timed(f"mixed(mode {args.mixed_mode}) block-Jacobi ILU0 PCG",
-                           args.mixed_mode)
+    tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0, steps=args.ref_runs)
 
     def errs(x):
-        wx = s.magnetic_energy(s.fill_phi(x))
-        return {"max_rel_dphi": float(np.max(np.abs(x - x64)) / np.max(np.abs(x64))),
-                "max_rel_denergy": float(np.max(np.abs(wx - w64) / np.abs(w64)))}
+        """phi, B and energy of x against the FP64 solve (max relative to the
+        largest magnitude; energy per component), asserted on the bars."""
+        phi = s.fill_phi(x)
+        dphi = float(np.max(np.abs(x - x64)) / np.max(np.abs(x64)))
+        B = s.getb(phi)
+        dB = max(float(np.max(np.abs(g - r)) / np.max(np.abs(r))) for g, r in zip(B, B64))
+        wx = s.magnetic_energy(phi)
+        dE = float(np.max(np.abs(wx - w64) / np.abs(w64)))
+        assert dphi <= PHI_TOL and dB <= B_TOL and dE <= E_TOL, (dphi, dB, dE)
+        return {"max_rel_dphi": dphi, "max_rel_dB": dB, "max_rel_denergy": dE}
 
+    # mixed precision from the zero guess
+    steps_mx = args.ref_runs if use_net else args.steps
+    tmx, stmx, xmx = timed(f"mixed(mode {args.mixed_mode}) block-Jacobi ILU0 PCG",
+                           args.mixed_mode, warmup=0 if use_net else args.warmup, steps=steps_mx)
+    err_mx = errs(xmx)
+    del xmx
+    tbest, stbest, vlabel, err_best = tmx, stmx, "mixed", err_mx
     result_ws = None
-    tbest, stbest, label = tmx, stmx, "mixed"
-    if ws is not None:
+    if use_net:
         from pot3d_amd import warmstart as W
-        wc = ws["wc"]
+        net, meta = W.WarmStartNet.load(wpath, wc.mesh)
         sc = solver.Pot3dSolver(wc.mesh, wc.pm_r1, 2)
-        sc.set_option(_lib.OPT_STOP, 1)
-        t0 = time.perf_counter()
-        xs, iters = W.generate_pairs(sc, ws["brs"])
-        t_pairs = time.perf_counter() - t0
-        print(f"[bench] {len(xs)} training solves ({iters} PCG iterations) in {t_pairs:.1f}s",
-              file=sys.stderr, flush=True)
-        net = W.WarmStartNet(wc.mesh, seed=0)
-        losses, t_train = W.train_network(net, ws["brs"], xs, epochs=args.ws_epochs,
-                                          lr=args.ws_lr, seed=1)
-        print(f"[bench] trained {losses.size} steps in {t_train:.1f}s: loss "
-              f"{float(np.mean(losses[:16])):.3e} -> {float(np.mean(losses[-16:])):.3e}",
-              file=sys.stderr, flush=True)
-        # the fine problem's br0 on the coarse mesh, and the coarse padded phi
-        from pot3d_amd import inputs
-        brc = inputs.synthetic_br0(wc.mesh, 0, n_bipoles=dom["n_bipoles"], option=dom["option"])
+        # the fine problem's Br map (seed 0) on the coarse mesh
+        brc = wc.br0
         sc.set_br0(brc)
         tab = W.prolong_tables(wc.mesh, w.mesh)
         d_phic = hip.malloc(8 * wc.mesh.nr * wc.mesh.nt * wc.mesh.np)
@@ -345,8 +378,9 @@ def main():
             W.prolong_dev(tab, d_phic, dx)
 
         twl, stwl, xwl = timed(f"mixed(mode {args.mixed_mode}) + learned block-Jacobi ILU0 PCG",
-                               args.mixed_mode, guess)
-        t64w, st64w, _ = timed("fp64 + learned block-Jacobi ILU0 PCG", 0, guess)
+                               args.mixed_mode, guess, warmup=args.warmup, steps=args.steps)
+        err_wl = errs(xwl)
+        del xwl
         # quality of the guess alone
         hip.h2d(d_x, zero)
         guess(d_x)
@@ -354,21 +388,18 @@ def main():
         hip.d2h(xg, d_x)
         g_err = float(np.linalg.norm(xg - x64) / np.linalg.norm(x64))
         g_res = float(np.linalg.norm(rhs - s.ax(xg)) / np.linalg.norm(rhs))
-        # A-norm error of the guess relative to the zero guess's
-        e = x64 - xg
-        g_anorm = float(np.sqrt(max(e @ s.ax(e), 0.0) / max(x64 @ s.ax(x64), 1e-300)))
+        del xg
         result_ws = {
-            "coarse_grid": args.ws_grid, "pairs": len(xs), "epochs": args.ws_epochs,
-            "params": net.nparams, "t_maps_s": round(ws["t_maps"], 2),
-            "t_pair_solves_s": round(t_pairs, 2), "t_train_s": round(t_train, 2),
-            "loss_first": float(np.mean(losses[:16])), "loss_last": float(np.mean(losses[-16:])),
-            "guess_rel_err": g_err, "guess_relres": g_res, "guess_rel_anorm_err": g_anorm,
-            "guess_check_mode": args.guess_check, "mixed_learned_s": round(min(twl), 4),
-            "mixed_learned_ncg": stwl["ncg"], "fp64_learned_s": round(min(t64w), 4),
-            "fp64_learned_ncg": st64w["ncg"], "guess_kept": stwl["guess"] == 1,
-            "errors_vs_fp64": errs(xwl)}
+            "weights": os.path.relpath(wpath, ROOT), "coarse_grid": "51x91x181",
+            "pairs": int(meta.get("pairs", 0)), "epochs": float(meta.get("epochs", 0)),
+            "val_guess_rel_err_coarse": float(meta.get("val_guess_rel_err", float("nan"))),
+            "params": net.nparams, "guess_rel_err": g_err, "guess_relres": g_res,
+            "guess_check_mode": args.guess_check, "guess_kept": stwl["guess"] == 1,
+            "mixed_learned_s": round(min(twl), 4), "mixed_learned_ncg": stwl["ncg"],
+            "iterations_saved_frac": round(1.0 - stwl["ncg"] / stmx["ncg"], 4),
+            "errors_vs_fp64": err_wl}
         if min(twl) < min(tbest):
-            tbest, stbest, label = twl, stwl, "mixed+learned"
+            tbest, stbest, vlabel, err_best = twl, stwl, "mixed+learned", err_wl
         net.close()
         sc.close()
         hip.free(d_phic)
@@ -376,17 +407,16 @@ def main():
     dk = vkern["dominant"]
     roof = {"kernel": dk["name"], "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(w.name, dk["name"]),
+            "traffic": measured_traffic(gname, dk["name"]),
             "avg_ms": dk["ms"], "bytes_per_launch": dk["bytes"]}
+    best = min(tbest)
+    ms_iter = 1e3 * best / max(stbest["ncg"], 1)
+    iter_roof = {"bytes_per_iteration": vkern["iter_bytes"], "ms_per_iteration": round(ms_iter, 4),
+                 "achieved": round(vkern["iter_bytes"] / (ms_iter * 1e-3) / 1e9, 1),
+                 "frac": round(vkern["iter_bytes"] / (ms_iter * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "kernels_ms_sum": round(vkern["kernels_ms"], 4)}
     st_name = "stencil_fp64 (k_stencil_march<double>)"
     per_iter64 = min(t64) / max(st64["ncg"], 1)
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(w, st64["ncg"], args.cpu_iters, min(16, os.cpu_count() or 1))
-        except Exception as e:  # reported, not fatal
-            cpu = {"error": str(e)}
-    best = min(tbest)
     out = {
         "metric": "PCG time-to-||r||/||b||<=1e-9 (s): mixed+learned vs FP64 plain; "
                   "stencil HBM GB/s",
@@ -394,32 +424,44 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(tbest)), 2),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 inner", "data": "synthetic (seeded bipole+harmonic Br)",
-        "config": {"workload": f"{'PFCS-style open' if args.large else 'PFSS'} {args.grid} "
-                               f"r in [{dom['r0']}, {dom['r1']}], {dom['n_bipoles']} bipoles, "
-                               f"ILU0 PCG to ||r||2/||b||2<=1e-9",
+        "config": {"workload": f"cfg{args.config} {label} {gname}, r in [{dom['r0']}, "
+                               f"{dom['r1']}], {dom['n_bipoles']} bipoles, option "
+                               f"'{dom['option']}', ILU0 PCG to ||r||2/||b||2<=1e-9",
                    "unknowns": N, "precond": "ILU0 (D-ILU); fast paths block-Jacobi ILU0",
-                   "epscg": 1e-9,
-                   "value_path": label, "mixed_mode": args.mixed_mode},
+                   "epscg": 1e-9, "value_path": vlabel, "mixed_mode": args.mixed_mode},
         "fp64_plain_s": round(min(t64), 4), "fp64_plain_ncg": st64["ncg"],
-        "ilu_blocks": list(blocks), "ilu_tile_k": args.tile_k, "ilu_tile_stage": args.tile_stage, "ilu_apply_blockjacobi": ilu_bj,
         "fp64_blockjacobi_s": round(min(tbj), 4), "fp64_blockjacobi_ncg": stbj["ncg"],
         "mixed_s": round(min(tmx), 4), "mixed_ncg": stmx["ncg"], "mixed_outer": stmx["n_outer"],
+        "value_ncg": stbest["ncg"],
         "speedup_vs_fp64": round(min(t64) / best, 3),
+        "speedup_same_precond": round(min(tbj) / best, 3),
+        "speedup_precision_only": round(min(tbj) / min(tmx), 3),
+        "ilu_blocks": list(blocks), "ilu_tile_k": args.tile_k,
+        "ilu_tile_stage": args.tile_stage,
         "fp64_levels_s": None if tlv is None else round(tlv[0], 4),
         "fp64_levels_ncg": None if tlv is None else tlv[1],
-        "speedup_vs_fp64_levels": None if tlv is None else round(tlv[0] / best, 3),
-        "mixed_errors_vs_fp64": errs(xmx),
+        "errors_vs_fp64": {"bars": [PHI_TOL, B_TOL, E_TOL], "mixed": err_mx,
+                           "value_path": err_best},
         "learned": result_ws,
         "fp64_ms_per_iter": round(1e3 * per_iter64, 4),
         "stencil_fp64_GBs": kern["table"][st_name]["GBs"],
         "kernels": kern["table"],
         "value_path_kernels": vkern["table"],
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "iteration_roofline": iter_roof,
     }
-    print(json.dumps(out))
     s.close()
     hip.free(d_x)
     hip.free(d_b)
+    del x64, B64, rhs
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline on {cores} cores", file=sys.stderr, flush=True)
+        try:
+            cpu = cpu_baseline(w, family, cores, args.cpu_iters, st64["ncg"])
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": repr(e)}
+    out["cpu_baseline"] = cpu
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -121,11 +121,19 @@ enum {
                                 over that many ranks (src/pot3d.F90:4940-
                                 4967); block extents are rounded up to 16,
                                 32 and 64 points. */
-  P3D_OPT_TILE_STAGE = 14   /* 8x8-column tiles: 0 planes/8 waves load each
+  P3D_OPT_TILE_STAGE = 14,  /* 8x8-column tiles: 0 planes/8 waves load each
                                 tile and one marches (default); 1 the
                                 marching wave loads its tile alone (one wave
                                 per workgroup, more tiles resident per CU).
                                 Same results bit for bit. */
+  P3D_OPT_STENCIL = 15,     /* operator kernel: 0 one thread per (i,j)
+                                column marching phi; 1 (default) each thread
+                                holds 8 theta rows of one column, the
+                                neighbours in i from the adjacent lanes.
+                                Same operator values bit for bit. */
+  P3D_OPT_APPLY_FP32 = 16   /* p3d_apply_A: 1 applies the FP32 operator of
+                                the mixed-precision recurrence (x rounded to
+                                float, the result in float); 0 FP64 */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /
@@ -179,6 +187,26 @@ int p3d_fill_phi(p3d_solver* s, const double* x, double* phi);
  * br/bt/bp may be NULL; else receive (nr-1,nt,np), (nr,nt-1,np),
  * (nr,nt,np-1). */
 int p3d_energy(p3d_solver* s, const double* phi, double* w, double* br, double* bt, double* bp);
+
+/* The arithmetic of the driver's readbr + set_flux on the GPU: replaces
+ * intrp2d/interp (src/pot3d.F90:6830-6910, 7028-7072), readbr's seam, pole
+ * and flux-balance steps (:6751-6829) and set_flux's |Br| (:3806-3833); the
+ * driver keeps rdhdf_2d and passes the map.  Interpolates bn(ntn,npn)
+ * (Fortran order, on the increasing scales tn(ntn), pn(npn)) bilinearly onto
+ * the half-mesh interior th(2:nt-1) x ph(2:np-1), sets br0(:,1) =
+ * br0(:,np-1) and br0(:,np) = br0(:,2), the m=0 pole ghosts br0(1,:) and
+ * br0(nt,:), and subtracts br00err = (fluxp+fluxm)/area from every point
+ * when balance_flux != 0 (the reference skips it for 'ss'/'open' with
+ * do_not_balance_flux); abs_br != 0 then takes |br0| (option 'open').
+ * Mesh arrays: th, sth, dth (nt) and ph, dph (np) of the global mesh
+ * (src/pot3d.F90:2583-2706); pl = phi period (p1-p0).  br0(nt,np) receives
+ * the result; flux (may be NULL) receives {fluxp, fluxm, area, br00err}.
+ * P3D_EINVAL for scales that do not increase (intrp2d's ierr=1) or a mesh
+ * point outside them (ierr=2).  Host arrays; synchronous. */
+int p3d_readbr(int nt, int np, const double* th, const double* ph, const double* sth,
+               const double* dth, const double* dph, double pl, int ntn, int npn,
+               const double* tn, const double* pn, const double* bn, int balance_flux,
+               int abs_br, double* br0, double* flux);
 
 /* Single operations (host vectors of N), for parity checks:
  * y = A x (ax, :5102-5150); x <- M^-1 x (prec_inv, :5152-5225). */
@@ -251,6 +279,13 @@ int p3d_nn_loss_grad(p3d_nn* n, const double* br0, const double* x, double* loss
  * seeded shuffled order; losses (nsteps floats, may be NULL) receive each
  * step's loss. */
 int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses);
+/* Training augmentation for p3d_nn_train, by exact symmetries of the
+ * discrete problem (all off by default): phi_roll rotates each sample by a
+ * random number of interior phi points (valid for a uniform phi mesh),
+ * theta_flip reflects it in theta at random (a theta mesh symmetric about
+ * the equator), sign_flip negates br0 and x at random (the solution is
+ * linear in br0).  The caller decides which hold for its mesh. */
+int p3d_nn_set_augment(p3d_nn* n, int phi_roll, int theta_flip, int sign_flip);
 /* x(N) predicted for br0(nt,np): host x, or device x_dev when non-NULL. */
 int p3d_nn_predict(p3d_nn* n, const double* br0, double* x, double* x_dev);
 /* Trilinear prolongation of a coarse padded field phic(nrc,ntc,npc) to the

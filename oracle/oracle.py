@@ -129,7 +129,7 @@ def _i(a):
 
 
 class Oracle:
-    """The oracle bound to one mesh (a pot3d_amd.inputs.Mesh) and pm_r1."""
+    """The oracle bound to one mesh (a p3d_driver.inputs.Mesh) and pm_r1."""
 
     def __init__(self, mesh, pm_r1: float):
         self.mesh = mesh

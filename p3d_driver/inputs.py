@@ -1,19 +1,21 @@
-"""Driver-side inputs for the solve: meshes and the r=1 Br boundary map.
+"""Driver-side inputs for the solve: meshes and the photospheric Br maps.
 
 POT3D's Fortran driver (not the solve path) builds the 1-D mesh arrays and
-reads/interpolates Br before calling the solver.  The solver's C-ABI takes
-those arrays as given.  This module restates the parts of the driver that
-the tests, the bench and the smoke run need in order to produce the same
-inputs the reference would hand to its solver:
+reads the Br map before calling the solver, whose C ABI takes them as given.
+This module restates the parts of the driver that the tests, the bench, the
+smoke run and the warm-start training need to produce the same inputs the
+reference would hand to its solver:
 
 * ``genmesh``            -- src/pot3d.F90:2864-3372 (+ map_xi_to_c :3374-3470,
                             map_c_to_xi :3472-3545, filters :3547-3700, fold)
 * ``global_mesh``        -- set_global_mesh, src/pot3d.F90:2583-2706
-* ``readbr_interp``      -- readbr + intrp2d, src/pot3d.F90:6672-6910
-* ``set_flux``           -- the 'open' |Br| flip, src/pot3d.F90:3806-3833
-* ``synthetic_br``       -- the seeded bipole + spherical-harmonic Br map
-                            used for the synthetic workloads (no reference
+* ``pm_r1_for``          -- the r1 boundary sign of each option (:1250-1285)
+* ``synthetic_br_map``   -- the seeded bipole + spherical-harmonic Br map
+                            of the synthetic workloads (no reference
                             counterpart; the reference reads an HDF5 map).
+
+readbr's interpolation, seam, pole and flux-balance arithmetic runs in the
+library (p3d_readbr); workloads.py hands it the maps.
 
 Everything is float64 and follows the reference's operation order where
 it affects rounding.
@@ -323,83 +325,6 @@ def geometric_r_ratio(per_cell_ratio: float, nr: int) -> float:
 # --------------------------------------------------------------------------
 # Br boundary map
 # --------------------------------------------------------------------------
-def _interp_table(x: np.ndarray, xv: np.ndarray):
-    """interp (src/pot3d.F90:7028-7072) for many points: the first i with
-    x(i) <= xv <= x(i+1), and alpha=(xv-x(i))/(x(i+1)-x(i))."""
-    x = np.asarray(x, float)
-    xv = np.asarray(xv, float)
-    if np.any(xv < x[0]) or np.any(xv > x[-1]):
-        raise ValueError("intrp2d: point outside the range of the scales")
-    # first interval containing xv: largest i with x[i] < xv, or i with x[i]==xv
-    i = np.searchsorted(x, xv, side="left") - 1
-    i = np.clip(i, 0, len(x) - 2)
-    alpha = (xv - x[i]) / (x[i + 1] - x[i])
-    return i, alpha
-
-
-def intrp2d(xi: np.ndarray, yi: np.ndarray, fi: np.ndarray, x: np.ndarray, y: np.ndarray):
-    """intrp2d: bilinear, src/pot3d.F90:6830-6910.  fi has shape (nxi,nyi);
-    the interpolation weights are separable, so they are computed per axis
-    and combined in the reference's expression order."""
-    if np.any(np.diff(xi) <= 0) or np.any(np.diff(yi) <= 0):
-        raise ValueError("intrp2d: scales are not monotonic")
-    ii, ax = _interp_table(xi, x)
-    jj, ay = _interp_table(yi, y)
-    ax = ax[:, None]
-    ay = ay[None, :]
-    f00 = fi[np.ix_(ii, jj)]
-    f01 = fi[np.ix_(ii, jj + 1)]
-    f10 = fi[np.ix_(ii + 1, jj)]
-    f11 = fi[np.ix_(ii + 1, jj + 1)]
-    return ((1.0 - ax) * ((1.0 - ay) * f00 + ay * f01)
-            + ax * ((1.0 - ay) * f10 + ay * f11))
-
-
-def readbr_interp(mesh: Mesh, tn: np.ndarray, pn: np.ndarray, bn: np.ndarray,
-                  option: str = "ss", do_not_balance_flux: bool = False) -> np.ndarray:
-    """readbr (src/pot3d.F90:6672-6829): interpolate a (theta,phi) map given on
-    scales tn, pn (bn shape (len(tn),len(pn))) onto th(2:ntm1) x ph(2:npm1),
-    make it periodic, set the pole m=0 ghosts and balance the flux.
-    Returns br0(nt,np) (Fortran index order, C array shape (nt,np))."""
-    nt, np_ = mesh.nt, mesh.np
-    br0 = np.zeros((nt, np_))
-    br0[1:nt - 1, 1:np_ - 1] = intrp2d(np.asarray(tn, float), np.asarray(pn, float),
-                                       np.asarray(bn, float), mesh.th[1:nt - 1],
-                                       mesh.ph[1:np_ - 1])
-    return _finish_br0(mesh, br0, option, do_not_balance_flux)
-
-
-def _finish_br0(mesh: Mesh, br0: np.ndarray, option: str, do_not_balance_flux: bool):
-    nt, np_ = mesh.nt, mesh.np
-    br0[:, 0] = br0[:, np_ - 2]
-    br0[:, np_ - 1] = br0[:, 1]
-    w = mesh.dph[1:np_ - 1]
-    sum0 = 0.0
-    sum1 = 0.0
-    for k in range(1, np_ - 1):  # Fortran SUM over k=2..npm1 in order
-        sum0 = sum0 + br0[1, k] * w[k - 1]
-        sum1 = sum1 + br0[nt - 2, k] * w[k - 1]
-    sum0 = sum0 * PL_I
-    sum1 = sum1 * PL_I
-    br0[0, :] = 2.0 * sum0 - br0[1, :]
-    br0[nt - 1, :] = 2.0 * sum1 - br0[nt - 2, :]
-    da = (mesh.sth[1:nt - 1, None] * mesh.dth[1:nt - 1, None]) * mesh.dph[None, 1:np_ - 1]
-    v = br0[1:nt - 1, 1:np_ - 1]
-    fluxp = float(np.sum(np.where(v > 0.0, v * da, 0.0)))
-    fluxm = float(np.sum(np.where(v > 0.0, 0.0, v * da)))
-    area = float(np.sum(da))
-    if not ((option in ("ss", "open")) and do_not_balance_flux):
-        br00err = (fluxp + fluxm) / area
-        br0 = br0 - br00err
-    return br0
-
-
-def set_flux(br0: np.ndarray, option: str) -> np.ndarray:
-    """set_flux: for option='open' use |Br| (the monopole trick,
-    src/pot3d.F90:3806-3833)."""
-    return np.abs(br0) if option == "open" else br0
-
-
 def pm_r1_for(option: str) -> float:
     """check_input: 'potential' -> +1 (dphi/dr=0 at r1); 'open'/'ss' -> -1
     (phi=0 at r1).  src/pot3d.F90:1250-1285."""
@@ -469,16 +394,3 @@ def synthetic_br_map(seed: int, n_bipoles: int = 8, ntheta: int = 181, nphi: int
     br += bg
     br -= float(np.sum(br * w) / np.sum(w))
     return th, ph, br
-
-
-def synthetic_br0(mesh: Mesh, seed: int, n_bipoles: int = 8, option: str = "ss",
-                  map_res_deg: float = 1.0) -> np.ndarray:
-    """br0(nt,np) for a mesh from the seeded synthetic map, passed through the
-    same interpolation/periodicity/pole/flux-balance steps as readbr and the
-    'open' sign flip of set_flux."""
-    nth = int(round(180.0 / map_res_deg)) + 1
-    nph = int(round(360.0 / map_res_deg)) + 1
-    tn, pn, bn = synthetic_br_map(seed, n_bipoles, nth, nph)
-    # the map's phi scale must cover ph(2:npm1); it spans [0,2pi].
-    br0 = readbr_interp(mesh, tn, pn, bn, option=option)
-    return set_flux(br0, option)

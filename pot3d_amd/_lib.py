@@ -33,6 +33,8 @@ SIGNATURES = {
     "p3d_solve_dev": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _ip, _dp]),
     "p3d_fill_phi": (C.c_int, [_vp, _dp, _dp]),
     "p3d_energy": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "p3d_readbr": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                             C.c_int, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),
     "p3d_apply_A": (C.c_int, [_vp, _dp, _dp]),
     "p3d_apply_prec": (C.c_int, [_vp, _dp]),
     "p3d_ilu_dinv": (C.c_int, [_vp, _fp]),
@@ -48,6 +50,7 @@ SIGNATURES = {
     "p3d_nn_add_sample": (C.c_int, [_vp, _dp, _dp]),
     "p3d_nn_loss_grad": (C.c_int, [_vp, _dp, _dp, _dp, _fp]),
     "p3d_nn_train": (C.c_int, [_vp, C.c_int, C.c_double, C.c_uint, _fp]),
+    "p3d_nn_set_augment": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "p3d_nn_predict": (C.c_int, [_vp, _dp, _dp, _vp]),
     "p3d_nn_prolong": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
                                  _ip, _dp, _ip, _dp, _ip, _dp, _dp]),
@@ -60,7 +63,7 @@ SIGNATURES = {
 P3D_OK, P3D_EINVAL, P3D_ENOMEM, P3D_EHIP, P3D_ENOCONV, P3D_ENONFINITE, P3D_ESTATE = range(7)
 OPT_STOP, OPT_MIXED, OPT_INNER_TOL, OPT_CHUNK, OPT_GRAPH, OPT_SWEEP, OPT_VERBOSE, \
     OPT_GUESS_CHECK, OPT_TILE_K, OPT_TILE_WAVES, OPT_ILU_BLOCKS_R, OPT_ILU_BLOCKS_T, \
-    OPT_ILU_BLOCKS_P, OPT_TILE_STAGE = range(1, 15)
+    OPT_ILU_BLOCKS_P, OPT_TILE_STAGE, OPT_STENCIL, OPT_APPLY_FP32 = range(1, 17)
 
 
 class Stats(C.Structure):

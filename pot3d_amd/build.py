@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpot3d_hip.so")
 SOURCES = ["p3d_kernels.hip", "p3d_sweep.hip", "p3d_solver.hip", "p3d_legacy.hip",
-           "p3d_nn_kernels.hip", "p3d_nn.hip"]
+           "p3d_nn_kernels.hip", "p3d_nn.hip", "p3d_readbr.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("P3D_OFFLOAD_ARCH", "gfx950")
 

@@ -36,9 +36,40 @@ struct Metric {
   const double* DPL;  // c
   const double* DPU;  // c
   const double* PW;   // c: dph(k)/(2 pi), pole-average weights
+  // DPH, DPL, DPU rounded to float: the per-plane factors of the FP32
+  // couplings of the ILU triangular solves (sweep_coup_*)
+  const float* FDPH;  // c
+  const float* FDPL;  // c
+  const float* FDPU;  // c
   int a, b, c;
   double pm_r1;       // r1 ghost factor (+1 Neumann, -1 Dirichlet)
 };
+
+// FP32 couplings of the D-ILU triangular solves.  A's off-diagonals split
+// into a per-column factor and a per-plane factor; each is rounded to float
+// once and the coupling is their float product:
+//   forward (lower)   k-1: K*FDPL[k]   j-1: J*FDPH[k]   i-1: I*FDPH[k]
+//                     wrap (k = c-1 to plane 0): K*FDPU[c-1]
+//   backward (upper)  k+1: K*FDPU[k]   j+1: J*FDPH[k]   i+1: I*FDPH[k]
+//                     wrap (k = 0 to plane c-1): K*FDPL[0]
+// with K = -(DRH CP), J = -(DRH CTL) or -(DRH CTH), I = -(SD CRL) or
+// -(SD CRH) in FP64 before the rounding.  A march step then forms a
+// coupling with one float multiply.  The factorisation (the pivots) uses
+// A's FP64 coefficients; every sweep schedule uses these couplings, so the
+// schedules agree bit for bit.
+struct SweepCoup {
+  float k, j, i;
+};
+__device__ __forceinline__ SweepCoup sweep_coup_lower(const Metric& m, int i, int j) {
+  const double drh = m.DRH[i];
+  return SweepCoup{(float)(-(drh * m.CP[j])), (float)(-(drh * m.CTL[j])),
+                   (float)(-(m.SD[j] * m.CRL[i]))};
+}
+__device__ __forceinline__ SweepCoup sweep_coup_upper(const Metric& m, int i, int j) {
+  const double drh = m.DRH[i];
+  return SweepCoup{(float)(-(drh * m.CP[j])), (float)(-(drh * m.CTH[j])),
+                   (float)(-(m.SD[j] * m.CRH[i]))};
+}
 
 struct Coef {
   double a1, a2, a3, a4, a5, a6, a7;

@@ -14,6 +14,8 @@
 //   getb + magnetic_energy :6187-6321 -> k_energy, k_getb
 #include <math.h>
 
+#include <algorithm>
+
 #include "p3d_kernels.hpp"
 
 namespace p3d {
@@ -187,46 +189,76 @@ __device__ __forceinline__ double pole_sum(const double* part, int pole, int i, 
 //   phi seam  periodic wrap                   (:5746-5758)
 // The coefficients are recomputed from the 1-D factor tables (no 56 B per
 // unknown coefficient stream).
+//
+// Coefficients a1..a7 of delsq (:5337-5374) at one point from its column
+// factors and its plane's factors, a4 = -(sum of the others).  Contraction
+// is off: every operator kernel forms them with the same roundings.
+__device__ __forceinline__ void op_coefs(double DRH, double CRL, double CRH, double SD,
+                                         double CTL, double CTH, double P1, double dph,
+                                         double dpl, double dpu, double (&a)[7]) {
+#pragma clang fp contract(off)
+  const double a1 = -(P1 * dpl);
+  const double a7 = -(P1 * dpu);
+  const double p2 = DRH * dph;
+  const double a2 = -(p2 * CTL);
+  const double a6 = -(p2 * CTH);
+  const double p3 = SD * dph;
+  const double a3 = -(p3 * CRL);
+  const double a5 = -(p3 * CRH);
+  a[0] = a1;
+  a[1] = a2;
+  a[2] = a3;
+  a[3] = -(a1 + a2 + a3 + a5 + a6 + a7);
+  a[4] = a5;
+  a[5] = a6;
+  a[6] = a7;
+}
+
+// The operator's value at one point from x and its six neighbours (xw/xe in
+// i, xs/xn in j, xkm/xkp in k; the ghost values replace the boundary
+// neighbours here), y or, with sub_b, b - y.  The sum is an explicit FMA
+// chain, so the result is the same bits in every operator kernel.
+//   FP64: the reference's delsq form, sum a_nb x_nb + a4 x.
+//   FP32 storage: the same operator with neighbour differences, formed and
+//   summed in double, only the result rounded to float.  For a smooth x,
+//   sum a_nb x_nb + a4 x cancels to far below its terms (about 1/h^2 for a
+//   Laplacian), and so does the sum of the differences (about 1/h); in
+//   float either loses that factor times eps32, which stalls the FP32
+//   Krylov recurrence on smooth (open-boundary, monopole-dominated)
+//   solutions.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-    k_stencil(Metric m, Grid3 g, int G, int CH, const T* __restrict__ x, T* __restrict__ y,
-              const T* __restrict__ b, int mode, const double* __restrict__ pole_part,
-              CgState* st, double* partial, unsigned* counter, Epi epi, double* out,
-              int guard, int want_dot) {
-  if (guard && st_done(st)) return;
-  const int Lb = logical_block(G);
-  const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
-  const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
-  const uint32_t ab = g.ab, a = (uint32_t)g.a;
-  const uint32_t wrap = (uint32_t)(g.c - 1) * ab;
-  double acc = 0.0;
-  for (uint32_t n = n0 + threadIdx.x; n < n1; n += kBlock) {
-    int i, j, k;
-    unflatten(g, n, i, j, k);
-    const Coef cf = coef_at(m, i, j, k);
-    const T xc = x[n];
-    const T xkm = (k > 0) ? x[n - ab] : x[n + wrap];
-    const T xkp = (k < g.c - 1) ? x[n + ab] : x[n - wrap];
-    const T xim = (i > 0) ? x[n - 1] : xc;
-    const T xip = (i < g.a - 1) ? x[n + 1] : (T)m.pm_r1 * xc;
-    T xjm, xjp;
-    if (j > 0)
-      xjm = x[n - a];
-    else
-      xjm = (T)(2.0 * pole_sum(pole_part, 0, i, g.a)) - xc;
-    if (j < g.b - 1)
-      xjp = x[n + a];
-    else
-      xjp = (T)(2.0 * pole_sum(pole_part, 1, i, g.a)) - xc;
-    T yv = (T)cf.a1 * xkm + (T)cf.a2 * xjm + (T)cf.a3 * xim + (T)cf.a4 * xc + (T)cf.a5 * xip +
-           (T)cf.a6 * xjp + (T)cf.a7 * xkp;
-    if (mode == 1) yv = b[n] - yv;
-    y[n] = yv;
-    acc += mode == 0 ? (double)xc * (double)yv : (double)yv * (double)yv;
+__device__ __forceinline__ T op_point(const double (&a)[7], T xkm, T xs, T xw, T xk, T xe, T xn,
+                                      T xkp, bool im, bool ip, bool jm, bool jp, double pm_r1,
+                                      double s0d, double s1d, bool sub_b, T bval) {
+#pragma clang fp contract(off)
+  if constexpr (sizeof(T) == 8) {
+    const T xim = im ? xw : xk;
+    const T xip = ip ? xe : (T)pm_r1 * xk;
+    const T xjm = jm ? xs : (T)s0d - xk;
+    const T xjp = jp ? xn : (T)s1d - xk;
+    T y = a[0] * xkm;
+    y = fma(a[1], xjm, y);
+    y = fma(a[2], xim, y);
+    y = fma(a[3], xk, y);
+    y = fma(a[4], xip, y);
+    y = fma(a[5], xjp, y);
+    y = fma(a[6], xkp, y);
+    return sub_b ? bval - y : y;
+  } else {
+    const double c = (double)xk;
+    const double dim = im ? (double)xw - c : 0.0;
+    const double dip = ip ? (double)xe - c : (pm_r1 - 1.0) * c;
+    const double djm = jm ? (double)xs - c : s0d - 2.0 * c;
+    const double djp = jp ? (double)xn - c : s1d - 2.0 * c;
+    double yd = a[0] * ((double)xkm - c);
+    yd = fma(a[1], djm, yd);
+    yd = fma(a[2], dim, yd);
+    yd = fma(a[4], dip, yd);
+    yd = fma(a[5], djp, yd);
+    yd = fma(a[6], (double)xkp - c, yd);
+    if (sub_b) yd = (double)bval - yd;
+    return (T)yd;
   }
-  if (!want_dot) return;
-  double v[1] = {acc};
-  block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
 }
 
 // 2.5-D form of the same operator: one thread per (i,j) column marching
@@ -282,10 +314,8 @@ __global__ void __launch_bounds__(kBlock)
     const double SD = m.SD[j], CTL = m.CTL[j], CTH = m.CTH[j];
     const double P1 = DRH * m.CP[j];
     const bool im = i > 0, ip = i < g.a - 1, jm = j > 0, jp = j < g.b - 1;
-    const T pmr = (T)m.pm_r1;
     const double s0d = jm ? 0.0 : 2.0 * pole_sum(pole_part, 0, i, g.a);
     const double s1d = jp ? 0.0 : 2.0 * pole_sum(pole_part, 1, i, g.a);
-    const T s0 = (T)s0d, s1 = (T)s1d;
     const uint32_t ab = g.ab, a = (uint32_t)g.a;
     const T* xc = x + mcol;
     T xkm = xc[(size_t)ab * (k0 > 0 ? k0 - 1 : g.c - 1)];
@@ -315,46 +345,10 @@ __global__ void __launch_bounds__(kBlock)
         if (k < k1) {
           const uint32_t n = mcol + ab * (uint32_t)k;
           const T xkp = xn[u];
-          const T xim = im ? xw[u] : xk;
-          const T xip = ip ? xe[u] : pmr * xk;
-          const T xjm = jm ? xs[u] : s0 - xk;
-          const T xjp = jp ? xnn[u] : s1 - xk;
-          const double dph = m.DPH[k];
-          const double a1 = -(P1 * m.DPL[k]);
-          const double a7 = -(P1 * m.DPU[k]);
-          const double p2 = DRH * dph;
-          const double a2 = -(p2 * CTL);
-          const double a6 = -(p2 * CTH);
-          const double p3 = SD * dph;
-          const double a3 = -(p3 * CRL);
-          const double a5 = -(p3 * CRH);
-          T yv;
-          if constexpr (sizeof(T) == 8) {
-            // the reference's delsq form (:5689-5760)
-            const double a4 = -(a1 + a2 + a3 + a5 + a6 + a7);
-            yv = (T)a1 * xkm + (T)a2 * xjm + (T)a3 * xim + (T)a4 * xk + (T)a5 * xip +
-                 (T)a6 * xjp + (T)a7 * xkp;
-            if (mode == 1) yv = bv[u] - yv;
-          } else {
-            // FP32 storage, FP64 arithmetic: the same operator written with
-            // neighbour differences (a4 = -sum of the others), formed and
-            // summed in double, only the result rounded to float.  For a
-            // smooth x, sum a_nb x_nb + a4 xk cancels to far below its terms
-            // (about 1/h^2 for a Laplacian), and so does the sum of the
-            // differences (about 1/h); in float either loses that factor
-            // times eps32, which stalls the FP32 Krylov recurrence on smooth
-            // (open-boundary, monopole-dominated) solutions.
-            // (ghost values at the boundaries in double as well)
-            const double c = (double)xk;
-            const double dim = im ? (double)xw[u] - c : 0.0;
-            const double dip = ip ? (double)xe[u] - c : (m.pm_r1 - 1.0) * c;
-            const double djm = jm ? (double)xs[u] - c : s0d - 2.0 * c;
-            const double djp = jp ? (double)xnn[u] - c : s1d - 2.0 * c;
-            double yd = a1 * ((double)xkm - c) + a2 * djm + a3 * dim + a5 * dip + a6 * djp +
-                        a7 * ((double)xkp - c);
-            if (mode == 1) yd = (double)bv[u] - yd;
-            yv = (T)yd;
-          }
+          double a[7];
+          op_coefs(DRH, CRL, CRH, SD, CTL, CTH, P1, m.DPH[k], m.DPL[k], m.DPU[k], a);
+          const T yv = op_point<T>(a, xkm, xs[u], xw[u], xk, xe[u], xnn[u], xkp, im, ip, jm, jp,
+                                   m.pm_r1, s0d, s1d, mode == 1, bv[u]);
           y[tmap(tm, g, n)] = yv;
           acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
           xkm = xk;
@@ -366,6 +360,168 @@ __global__ void __launch_bounds__(kBlock)
   if (!want_dot) return;
   double v[1] = {acc};
   block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
+}
+
+// ---------------------------------------------------------------------------
+// Row-blocked operator march (Launch::stencil = 1): the operator and the
+// per-point arithmetic of k_stencil_march, with the work laid out so that
+// a point costs one coalesced row load of x instead of five neighbour loads.
+// Each wave owns 64 consecutive i (one per lane) by kRows theta rows and
+// marches a segment of phi planes.  Lanes 1..62 produce output; lanes 0 and
+// 63 only load the i-1 / i+1 neighbours of the wave's edge columns.  Per
+// plane a thread loads its kRows rows plus the halo rows j0-1 and j0+kRows
+// (coalesced 64-lane rows); the i neighbours come from the adjacent lanes
+// by DPP wave shifts, the j neighbours from the thread's own rows and the k
+// neighbours from the march, which keeps planes k-1..k+1 in registers and
+// the loads of plane k+2 in flight.  A workgroup's four waves take four
+// consecutive row blocks, so their halo rows are each other's rows.
+constexpr int kRows = 8;
+constexpr int kRowLanes = 62;
+constexpr int kWaveShr1 = 0x138;  // DPP: lane l reads lane l-1
+constexpr int kWaveShl1 = 0x130;  // DPP: lane l reads lane l+1
+
+template <int CTRL>
+__device__ __forceinline__ float lane_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double lane_shift(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__host__ __device__ inline void stencil_rows_shape(const Grid3& g, int& nwi, int& nwg) {
+  nwi = (g.a + kRowLanes - 1) / kRowLanes;
+  nwg = ((g.b + kRows - 1) / kRows + 3) / 4;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_stencil_rows(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
+                   const T* __restrict__ b, int mode, const double* __restrict__ pole_part,
+                   CgState* st, double* partial, unsigned* counter, Epi epi, double* out,
+                   int guard, int want_dot, TileMap tm) {
+  if (guard && st_done(st)) return;
+  const int G = gridDim.x;
+  const int Lb = logical_block(G);
+  int nwi, nwg;
+  stencil_rows_shape(g, nwi, nwg);
+  // row groups fastest (neighbouring groups share halo rows in L2), then
+  // i blocks, then phi segments
+  const int seg = Lb / (nwi * nwg), rem = Lb - seg * (nwi * nwg);
+  const int wi = rem / nwg, wg = rem - wi * nwg;
+  const int kper = (g.c + nseg - 1) / nseg;
+  const int k0 = seg * kper, k1 = min(g.c, k0 + kper);
+  // the wave index as a wave-uniform value, so that the row factors and
+  // per-plane factors below are scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int j0 = (wg * 4 + wave) * kRows;
+  double acc = 0.0;
+  // wave-uniform: every lane of a working wave runs the loads and the DPP
+  if (seg < nseg && j0 < g.b && k0 < k1) {
+    const int i = wi * kRowLanes - 1 + lane;
+    const bool out_lane = lane >= 1 && lane <= kRowLanes && i < g.a;
+    const int il = min(max(i, 0), g.a - 1);
+    const double DRH = m.DRH[il], CRL = m.CRL[il], CRH = m.CRH[il];
+    const bool im = i > 0, ip = i < g.a - 1;
+    const double s0d = j0 == 0 ? 2.0 * pole_sum(pole_part, 0, il, g.a) : 0.0;
+    const double s1d = j0 + kRows >= g.b ? 2.0 * pole_sum(pole_part, 1, il, g.a) : 0.0;
+    const uint32_t ab = g.ab, a_ = (uint32_t)g.a;
+    // x offsets of the thread's rows j0-1 .. j0+kRows (clamped into the grid;
+    // clamped rows are never used)
+    uint32_t ro[kRows + 2];
+#pragma unroll
+    for (int r = 0; r < kRows + 2; r++) {
+      const int j = min(max(j0 - 1 + r, 0), g.b - 1);
+      ro[r] = (uint32_t)il + a_ * (uint32_t)j;
+    }
+    auto load = [&](T (&buf)[kRows + 2], int k) {
+      const int kw = ((k % g.c) + g.c) % g.c;  // phi periodic (lookahead may pass c)
+      const T* xp = x + (size_t)ab * (uint32_t)kw;
+#pragma unroll
+      for (int r = 0; r < kRows + 2; r++) buf[r] = xp[ro[r]];
+    };
+    auto plane = [&](const T (&pv)[kRows + 2], const T (&cv)[kRows + 2],
+                     const T (&nv)[kRows + 2], int k) {
+      const double dph = m.DPH[k], dpl = m.DPL[k], dpu = m.DPU[k];
+#pragma unroll
+      for (int r = 0; r < kRows; r++) {
+        const int j = j0 + r;
+        if (j < g.b) {
+          const T xk = cv[r + 1];
+          const T xl = lane_shift<kWaveShr1>(xk);
+          const T xr = lane_shift<kWaveShl1>(xk);
+          const bool jm = j > 0, jp = j < g.b - 1;
+          const double SD = m.SD[j], CTL = m.CTL[j], CTH = m.CTH[j];
+          const double P1 = DRH * m.CP[j];
+          const T xkm = pv[r + 1], xkp = nv[r + 1];
+          double a[7];
+          op_coefs(DRH, CRL, CRH, SD, CTL, CTH, P1, dph, dpl, dpu, a);
+          const uint32_t n = (uint32_t)il + a_ * (uint32_t)j + ab * (uint32_t)k;
+          const T bval = (mode == 1 && out_lane) ? b[n] : (T)0;
+          const T yv = op_point<T>(a, xkm, cv[r], xl, xk, xr, cv[r + 2], xkp, im, ip, jm, jp,
+                                   m.pm_r1, s0d, s1d, mode == 1, bval);
+          if (out_lane) {
+            y[tmap(tm, g, n)] = yv;
+            acc += mode == 0 ? (double)xk * (double)yv : (double)yv * (double)yv;
+          }
+        }
+      }
+    };
+    // four plane buffers in rotation: planes k-1, k, k+1 and k+2 (loading)
+    T B0[kRows + 2], B1[kRows + 2], B2[kRows + 2], B3[kRows + 2];
+    load(B0, k0 - 1);
+    load(B1, k0);
+    load(B2, k0 + 1);
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      load(B3, k + 2);
+      plane(B0, B1, B2, k);
+      load(B0, k + 3);
+      plane(B1, B2, B3, k + 1);
+      load(B1, k + 4);
+      plane(B2, B3, B0, k + 2);
+      load(B2, k + 5);
+      plane(B3, B0, B1, k + 3);
+    }
+    // the remaining 0..3 planes (buffers B0, B1, B2 hold planes k-1, k, k+1)
+    if (k < k1) {
+      load(B3, k + 2);
+      plane(B0, B1, B2, k);
+    }
+    if (k + 1 < k1) {
+      load(B0, k + 3);
+      plane(B1, B2, B3, k + 1);
+    }
+    if (k + 2 < k1) {
+      load(B1, k + 4);
+      plane(B2, B3, B0, k + 2);
+    }
+  }
+  if (!want_dot) return;
+  double v[1] = {acc};
+  block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
+}
+
+static int stencil_rows_segments(const Grid3& g) {
+  // about 16 workgroups (64 waves) per CU in total, >= kSegPlanes planes each
+  int nwi, nwg;
+  stencil_rows_shape(g, nwi, nwg);
+  int nseg = (16 * 256 + nwi * nwg - 1) / (nwi * nwg);
+  const int maxseg = (g.c + kSegPlanes - 1) / kSegPlanes;
+  if (nseg > maxseg) nseg = maxseg;
+  if (nseg < 1) nseg = 1;
+  return nseg;
+}
+
+static int stencil_rows_blocks(const Grid3& g) {
+  int nwi, nwg;
+  stencil_rows_shape(g, nwi, nwg);
+  const int G = nwi * nwg * stencil_rows_segments(g);
+  return (G + 7) / 8 * 8;
 }
 
 static int stencil_segments(const Grid3& g, const TileMap& tm) {
@@ -380,7 +536,7 @@ static int stencil_segments(const Grid3& g, const TileMap& tm) {
 
 int stencil_blocks(const Grid3& g, const TileMap& tm) {
   const int G = stencil_col_blocks(g, tm) * stencil_segments(g, tm);
-  return (G + 7) / 8 * 8;
+  return std::max((G + 7) / 8 * 8, stencil_rows_blocks(g));
 }
 
 template <typename T>
@@ -388,10 +544,19 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     const T* b, int mode, const double* pole_part, CgState* st, Reduce red,
                     int slot, Epi epi, double* out, bool guard, hipStream_t s,
                     const TileMap& tm) {
-  (void)L;
   const int want = (epi != EPI_NONE);
+  if (L.stencil == 1) {
+    const int nseg = stencil_rows_segments(g);
+    const int G = stencil_rows_blocks(g);
+    k_stencil_rows<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
+                                            red.partial,
+                                            red.counter + (size_t)slot * kRedCounters, epi, out,
+                                            guard ? 1 : 0, want, tm);
+    return;
+  }
   const int nseg = stencil_segments(g, tm);
-  const int G = stencil_blocks(g, tm);
+  const int G0 = stencil_col_blocks(g, tm) * nseg;
+  const int G = (G0 + 7) / 8 * 8;
   k_stencil_march<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,
                                            red.partial, red.counter + (size_t)slot * kRedCounters,
                                            epi, out, guard ? 1 : 0, want, tm);
@@ -826,6 +991,29 @@ struct MFCoefs {
   }
 };
 
+// Couplings of the level-scheduled triangular solves: the FP32 sweep
+// couplings (sweep_coup_*), as doubles for the providers' interface (the
+// factorisation keeps MFCoefs' FP64 coefficients).
+struct MFSweep {
+  Metric m;
+  __device__ __forceinline__ void lower(int i, int j, int k, uint32_t, double& ak, double& aj,
+                                        double& ai, double& aw) const {
+    const SweepCoup c = sweep_coup_lower(m, i, j);
+    ak = (double)(c.k * m.FDPL[k]);
+    aj = (double)(c.j * m.FDPH[k]);
+    ai = (double)(c.i * m.FDPH[k]);
+    aw = (double)(c.k * m.FDPU[m.c - 1]);  // at k=c-1: coupling to (i,j,0)
+  }
+  __device__ __forceinline__ void upper(int i, int j, int k, uint32_t, double& bi, double& bj,
+                                        double& bk, double& bw) const {
+    const SweepCoup c = sweep_coup_upper(m, i, j);
+    bi = (double)(c.i * m.FDPH[k]);
+    bj = (double)(c.j * m.FDPH[k]);
+    bk = (double)(c.k * m.FDPU[k]);
+    bw = (double)(c.k * m.FDPL[0]);  // at k=0: coupling to (i,j,c-1)
+  }
+};
+
 struct StoredCoefs {
   CoefArrays ca;
   const float* dg;
@@ -1011,7 +1199,7 @@ void launch_dilu_solve_levels(const Metric& m, const Grid3& g, const float* dinv
     cp.ab = g.ab;
     dilu_solve_levels<T>(cp, g, no_blocks(g, wrap), dinv, r, yv, z, guard, s);
   } else {
-    MFCoefs cp;
+    MFSweep cp;
     cp.m = m;
     dilu_solve_levels<T>(cp, g, bl, dinv, r, yv, z, guard, s);
   }

@@ -89,6 +89,7 @@ __device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, d
 struct Launch {
   int G;        // blocks of a streaming kernel (multiple of 8)
   int CH;       // points per block (multiple of 256)
+  int stencil = 0;  // operator kernel: 0 column march, 1 row-blocked march
 };
 
 constexpr int kBlock = 256;

@@ -66,6 +66,10 @@ struct p3d_nn {
   std::vector<double> ds_bs;  // per-sample input scale
   std::vector<float*> ds_t;
   float* tgt = nullptr;  // current target (FP32, /phiscale)
+  // training augmentation by exact symmetries of the discrete problem:
+  // rotation in phi (uniform phi mesh), reflection in theta (mesh symmetric
+  // about the equator), sign (the solution is linear in br0)
+  int aug_roll = 0, aug_flip = 0, aug_sign = 0;
   std::vector<void*> allocs;
 };
 
@@ -151,8 +155,10 @@ static void backward(p3d_nn* n) {
            s);
 }
 
-static void make_features(p3d_nn* n, const double* br_dev, double bs) {
-  features(n->D[0], 16, n->nt, br_dev, n->decay, n->rpos, (float)(1.0 / bs), n->x0, n->stream);
+static void make_features(p3d_nn* n, const double* br_dev, double bs, int roll = 0,
+                          int flip = 0) {
+  features(n->D[0], 16, n->nt, br_dev, n->decay, n->rpos, (float)(1.0 / bs), roll, flip, n->x0,
+           n->stream);
 }
 
 // Input scale of one br0(nt,np) (host): the fixed bscale, or with bscale = 0
@@ -384,9 +390,21 @@ int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses)
     }
     const int q = order.back();
     order.pop_back();
-    make_features(n, n->ds_br[q], n->ds_bs[q]);
+    const float* tq = n->ds_t[q];
+    double bs = n->ds_bs[q];
+    int roll = 0, flip = 0;
+    if (n->aug_roll || n->aug_flip || n->aug_sign) {
+      const uint64_t u = rng();
+      roll = n->aug_roll ? (int)((u >> 8) % (uint64_t)n->c) : 0;
+      flip = n->aug_flip ? (int)(u & 1) : 0;
+      const float sign = (n->aug_sign && (u & 2)) ? -1.f : 1.f;
+      bs *= sign;
+      gather_target(n->D[0], roll, flip, sign, tq, n->tgt, n->stream);
+      tq = n->tgt;
+    }
+    make_features(n, n->ds_br[q], bs, roll, flip);
     forward(n);
-    mse(V, 16, n->out, n->ds_t[q], n->g_out, n->lpart, dloss + it, n->stream);
+    mse(V, 16, n->out, tq, n->g_out, n->lpart, dloss + it, n->stream);
     backward(n);
     n->step++;
     adam(n->nparam, n->w, n->g, n->m1, n->m2, (float)lr, 0.9f, 0.999f, 1e-8f, n->step, n->wb,
@@ -397,6 +415,15 @@ int p3d_nn_train(p3d_nn* n, int nsteps, double lr, unsigned seed, float* losses)
     e = hipMemcpy(losses, dloss, sizeof(float) * nsteps, hipMemcpyDeviceToHost);
   hipFree(dloss);
   if (e != hipSuccess) return nfail(P3D_EHIP, "training failed");
+  return P3D_OK;
+}
+
+// Training augmentation switches (see struct p3d_nn); all off by default.
+int p3d_nn_set_augment(p3d_nn* n, int phi_roll, int theta_flip, int sign_flip) {
+  if (!n) return nfail(P3D_EINVAL, "null argument");
+  n->aug_roll = phi_roll != 0;
+  n->aug_flip = theta_flip != 0;
+  n->aug_sign = sign_flip != 0;
   return P3D_OK;
 }
 
@@ -427,20 +454,28 @@ int p3d_nn_predict(p3d_nn* n, const double* br0, double* x, double* x_dev) {
 // fine interior points (af x bf x cf) with per-axis tables (host): lower
 // coarse padded index and upper weight per fine interior index.  dev: phic
 // and xf are device addresses (else host arrays).
-static int prolong_impl(int nrc, int ntc, int npc, const double* phic, int af, int bf, int cf,
-                        const int* ir, const double* wr, const int* it, const double* wt,
-                        const int* ip, const double* wp, double* xf, bool dev) {
+static int prolong_check(int nrc, int ntc, int npc, const double* phic, int af, int bf, int cf,
+                         const int* ir, const double* wr, const int* it, const double* wt,
+                         const int* ip, const double* wp, const double* xf) {
   if (!phic || !ir || !wr || !it || !wt || !ip || !wp || !xf)
     return nfail(P3D_EINVAL, "null argument");
   if (nrc < 2 || ntc < 2 || npc < 2 || af < 1 || bf < 1 || cf < 1)
     return nfail(P3D_EINVAL, "bad grid sizes");
-  const size_t nc = (size_t)nrc * ntc * npc, nf = (size_t)af * bf * cf;
   for (int q = 0; q < af; q++)
     if (ir[q] < 0 || ir[q] + 1 >= nrc) return nfail(P3D_EINVAL, "r table out of range");
   for (int q = 0; q < bf; q++)
     if (it[q] < 0 || it[q] + 1 >= ntc) return nfail(P3D_EINVAL, "theta table out of range");
   for (int q = 0; q < cf; q++)
     if (ip[q] < 0 || ip[q] + 1 >= npc) return nfail(P3D_EINVAL, "phi table out of range");
+  return P3D_OK;
+}
+
+static int prolong_impl(int nrc, int ntc, int npc, const double* phic, int af, int bf, int cf,
+                        const int* ir, const double* wr, const int* it, const double* wt,
+                        const int* ip, const double* wp, double* xf, bool dev) {
+  const int rc = prolong_check(nrc, ntc, npc, phic, af, bf, cf, ir, wr, it, wt, ip, wp, xf);
+  if (rc != P3D_OK) return rc;
+  const size_t nc = (size_t)nrc * ntc * npc, nf = (size_t)af * bf * cf;
   const size_t nt_ = (size_t)af + bf + cf;
   double *dc = nullptr, *df = nullptr, *dw = nullptr;
   int* di = nullptr;

@@ -34,8 +34,13 @@ void adam(size_t n, float* w, const float* g, float* m1, float* m2, float lr, fl
           float b2, float eps, int step, uint16_t* wb, hipStream_t s);
 void to_bf16(size_t n, const float* x, uint16_t* y, hipStream_t s);
 void fill_u16(size_t n, uint16_t* p, uint16_t v, hipStream_t s);
+// roll/flip: the sample rotated by roll interior phi points and reflected in
+// theta (training augmentation; 0, 0 otherwise)
 void features(const Dims3& d, int C, int nt, const double* br0, const float* decay,
-              const float* rpos, float inv_bscale, uint16_t* x0, hipStream_t s);
+              const float* rpos, float inv_bscale, int roll, int flip, uint16_t* x0,
+              hipStream_t s);
+void gather_target(const Dims3& d, int roll, int flip, float sign, const float* src, float* dst,
+                   hipStream_t s);
 void head_to_x(int V, int C, const float* out, double scale, double* xo, hipStream_t s);
 void prolong(int af, int bf, int cf, int nrc, int ntc, int npc, const double* phic,
              const int* ir, const double* wr, const int* it, const double* wt, const int* ip,

@@ -352,17 +352,34 @@ __global__ void k_fill_u16(size_t n, uint16_t* p, uint16_t v) {
 //   ch0 = br0 * (r0/r)^2 / bscale,  ch1 = br0 / bscale,  ch2 = (r-r0)/(r1-r0)
 __global__ void k_features(Dims d, int C, int nt, const double* __restrict__ br0,
                            const float* __restrict__ decay, const float* __restrict__ rpos,
-                           float inv_bscale, uint16_t* __restrict__ x0) {
+                           float inv_bscale, int roll, int flip, uint16_t* __restrict__ x0) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= d.V) return;
   int i, j, k;
   coords(d, v, i, j, k);
-  const float b = (float)br0[(size_t)(j + 1) + (size_t)nt * (k + 1)] * inv_bscale;
+  // the sample seen rotated by `roll` interior phi points and, with flip,
+  // reflected in theta (j -> Y-1-j); both 0 outside training
+  const int js = flip ? d.Y - 1 - j : j;
+  const int ks = k + roll < d.Z ? k + roll : k + roll - d.Z;
+  const float b = (float)br0[(size_t)(js + 1) + (size_t)nt * (ks + 1)] * inv_bscale;
   uint16_t* o = x0 + (size_t)v * C;
   o[0] = f2bf(b * decay[i]);
   o[1] = f2bf(b);
   o[2] = f2bf(rpos[i]);
   for (int c = 3; c < C; c++) o[c] = 0;
+}
+
+// Training target of a rotated/reflected sample: dst(i,j,k) = sign *
+// src(i, flip ? Y-1-j : j, (k + roll) mod Z), the same map as k_features.
+__global__ void k_gather_target(Dims d, int roll, int flip, float sign,
+                                const float* __restrict__ src, float* __restrict__ dst) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= d.V) return;
+  int i, j, k;
+  coords(d, v, i, j, k);
+  const int js = flip ? d.Y - 1 - j : j;
+  const int ks = k + roll < d.Z ? k + roll : k + roll - d.Z;
+  dst[v] = sign * src[i + d.X * (js + d.Y * ks)];
 }
 
 // x (FP64, packed) = scale * out[v][0]
@@ -491,9 +508,16 @@ void fill_u16(size_t n, uint16_t* p, uint16_t v, hipStream_t s) {
   k_fill_u16<<<nblk(n, 256), 256, 0, s>>>(n, p, v);
 }
 void features(const Dims3& d3, int C, int nt, const double* br0, const float* decay,
-              const float* rpos, float inv_bscale, uint16_t* x0, hipStream_t s) {
+              const float* rpos, float inv_bscale, int roll, int flip, uint16_t* x0,
+              hipStream_t s) {
   Dims d{d3.X, d3.Y, d3.Z, d3.X * d3.Y * d3.Z};
-  k_features<<<nblk(d.V, 256), 256, 0, s>>>(d, C, nt, br0, decay, rpos, inv_bscale, x0);
+  k_features<<<nblk(d.V, 256), 256, 0, s>>>(d, C, nt, br0, decay, rpos, inv_bscale, roll, flip,
+                                             x0);
+}
+void gather_target(const Dims3& d3, int roll, int flip, float sign, const float* src, float* dst,
+                   hipStream_t s) {
+  Dims d{d3.X, d3.Y, d3.Z, d3.X * d3.Y * d3.Z};
+  k_gather_target<<<nblk(d.V, 256), 256, 0, s>>>(d, roll, flip, sign, src, dst);
 }
 void head_to_x(int V, int C, const float* out, double scale, double* xo, hipStream_t s) {
   k_head_to_x<<<nblk(V, 256), 256, 0, s>>>(V, C, out, scale, xo);

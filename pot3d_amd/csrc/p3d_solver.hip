@@ -68,6 +68,7 @@ struct p3d_solver {
   hipStream_t stream;
   // device memory
   double* tables = nullptr;
+  float* ftables = nullptr;  // FDPH | FDPL | FDPU
   double* mesh = nullptr;  // dr, rh, drh, dt, dth, sth, dp, dph (full length)
   EnergyMesh em;
   double* br0 = nullptr;
@@ -107,6 +108,7 @@ struct p3d_solver {
   int verbose = 0;
   int guess_check = 0;
   int tile_k = 32, tile_waves = 1, tile_stage = 0;
+  int apply_fp32 = 0;  // p3d_apply_A with the FP32 operator
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
   // graphs: [0] FP64, [1] FP32, [2] FP32 recurrence with the FP64 solution
@@ -124,6 +126,12 @@ static int dalloc(void** p, size_t bytes) {
     return fail(P3D_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
   }
   return P3D_OK;
+}
+
+template <typename A, typename B>
+__global__ void k_convert(uint32_t n, const A* __restrict__ src, B* __restrict__ dst) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = (B)src[q];
 }
 
 static int ensure_mixed(p3d_solver* s) {
@@ -656,7 +664,7 @@ int p3d_destroy(p3d_solver* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   for (int t = 0; t < 3; t++)
     if (s->gexec[t]) hipGraphExecDestroy(s->gexec[t]);
-  void* bufs[] = {s->tables, s->mesh, s->br0, s->br1, s->x, s->r, s->p, s->ap, s->z, s->b,
+  void* bufs[] = {s->tables, s->ftables, s->mesh, s->br0, s->br1, s->x, s->r, s->p, s->ap, s->z, s->b,
                   s->yv, s->dinv, s->x32, s->r32, s->p32, s->ap32, s->z32, s->pole_part,
                   s->st, s->hist, s->red.partial, s->red.counter, s->out, s->phi, s->sums,
                   s->e_partial};
@@ -767,6 +775,22 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   s->m.DPL = dT + 3 * a + 4 * b + c;
   s->m.DPU = dT + 3 * a + 4 * b + 2 * c;
   s->m.PW = dT + 3 * a + 4 * b + 3 * c;
+  {
+    // float copies of the per-plane factors for the ILU sweeps' couplings
+    std::vector<float> F(3 * c);
+    for (uint64_t k = 0; k < c; k++) {
+      F[k] = (float)DPH[k];
+      F[c + k] = (float)DPL[k];
+      F[2 * c + k] = (float)DPU[k];
+    }
+    if ((rc = dalloc((void**)&s->ftables, F.size() * sizeof(float)))) return bail(rc);
+    if (hipMemcpy(s->ftables, F.data(), F.size() * sizeof(float), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return bail(fail(P3D_EHIP, "copy of mesh tables failed"));
+    s->m.FDPH = s->ftables;
+    s->m.FDPL = s->ftables + c;
+    s->m.FDPU = s->ftables + 2 * c;
+  }
   s->m.a = (int)a;
   s->m.b = (int)b;
   s->m.c = (int)c;
@@ -934,6 +958,23 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       return update_tilemap(s);
     case P3D_OPT_VERBOSE:
       s->verbose = value != 0;
+      break;
+    case P3D_OPT_STENCIL: {
+      const int v = (int)value;
+      if (v != 0 && v != 1) return fail(P3D_EINVAL, "P3D_OPT_STENCIL must be 0 or 1");
+      if (v != s->L.stencil) {
+        // captured iterations hold the other kernel
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < 3; q++) {
+          if (s->gexec[q]) hipGraphExecDestroy(s->gexec[q]);
+          s->gexec[q] = nullptr;
+        }
+        s->L.stencil = v;
+      }
+      break;
+    }
+    case P3D_OPT_APPLY_FP32:
+      s->apply_fp32 = value != 0;
       break;
     case P3D_OPT_TILE_K:
     case P3D_OPT_TILE_WAVES:
@@ -1148,9 +1189,31 @@ int p3d_apply_A(p3d_solver* s, const double* x, double* y) {
   if (!s || !x || !y) return fail(P3D_EINVAL, "null argument");
   const size_t nb = sizeof(double) * s->g.N;
   HIPCHK(hipMemcpyAsync(s->p, x, nb, hipMemcpyHostToDevice, s->stream));
-  launch_pole_sums<double>(s->m, s->g, s->p, s->pole_part, s->stream);
-  launch_stencil<double>(s->m, s->g, s->L, s->p, s->ap, nullptr, 0, s->pole_part, nullptr,
-                         s->red, 0, EPI_NONE, nullptr, false, s->stream);
+  if (s->apply_fp32) {
+    // the FP32 operator of the mixed recurrence, natural-order output
+    float *xf = nullptr, *yf = nullptr;
+    int rc;
+    if ((rc = dalloc((void**)&xf, s->g.N * sizeof(float))) ||
+        (rc = dalloc((void**)&yf, s->g.N * sizeof(float)))) {
+      hipFree(xf);
+      return rc;
+    }
+    const int nblk = (int)((s->g.N + kBlock - 1) / kBlock);
+    k_convert<double, float><<<nblk, kBlock, 0, s->stream>>>(s->g.N, s->p, xf);
+    launch_pole_sums<float>(s->m, s->g, xf, s->pole_part, s->stream);
+    launch_stencil<float>(s->m, s->g, s->L, xf, yf, nullptr, 0, s->pole_part, nullptr, s->red, 0,
+                          EPI_NONE, nullptr, false, s->stream);
+    k_convert<float, double><<<nblk, kBlock, 0, s->stream>>>(s->g.N, yf, s->ap);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(xf);
+    hipFree(yf);
+    if (e != hipSuccess) return fail(P3D_EHIP, "FP32 operator: %s", hipGetErrorString(e));
+  } else {
+    launch_pole_sums<double>(s->m, s->g, s->p, s->pole_part, s->stream);
+    launch_stencil<double>(s->m, s->g, s->L, s->p, s->ap, nullptr, 0, s->pole_part, nullptr,
+                           s->red, 0, EPI_NONE, nullptr, false, s->stream);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(y, s->ap, nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));

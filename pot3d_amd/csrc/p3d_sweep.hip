@@ -65,29 +65,22 @@ __device__ __forceinline__ void lanes_from_upper(float t, float& from_i, float& 
 // the solver) or stored FP32 couplings (the legacy CSR input).
 struct TileMF {
   Metric m;
-  double DRH, CRL, CRH, SD, CTL, CTH, P1;
+  SweepCoup lo, up;
   __device__ __forceinline__ void init(int ic, int jc, uint32_t) {
-    DRH = m.DRH[ic];
-    CRL = m.CRL[ic];
-    CRH = m.CRH[ic];
-    SD = m.SD[jc];
-    CTL = m.CTL[jc];
-    CTH = m.CTH[jc];
-    P1 = DRH * m.CP[jc];
+    lo = sweep_coup_lower(m, ic, jc);
+    up = sweep_coup_upper(m, ic, jc);
   }
-  __device__ __forceinline__ float aw(uint32_t) const { return (float)(-(P1 * m.DPU[m.c - 1])); }
-  __device__ __forceinline__ float bw(uint32_t) const { return (float)(-(P1 * m.DPL[0])); }
+  __device__ __forceinline__ float aw(uint32_t) const { return lo.k * m.FDPU[m.c - 1]; }
+  __device__ __forceinline__ float bw(uint32_t) const { return up.k * m.FDPL[0]; }
   __device__ __forceinline__ void lower(int k, uint32_t, float& ak, float& aj, float& ai) const {
-    const double dph = m.DPH[k];
-    ak = (float)(-(P1 * m.DPL[k]));
-    aj = (float)(-((DRH * dph) * CTL));
-    ai = (float)(-((SD * dph) * CRL));
+    ak = lo.k * m.FDPL[k];
+    aj = lo.j * m.FDPH[k];
+    ai = lo.i * m.FDPH[k];
   }
   __device__ __forceinline__ void upper(int k, uint32_t, float& bk, float& bj, float& bi) const {
-    const double dph = m.DPH[k];
-    bk = (float)(-(P1 * m.DPU[k]));
-    bj = (float)(-((DRH * dph) * CTH));
-    bi = (float)(-((SD * dph) * CRH));
+    bk = up.k * m.FDPU[k];
+    bj = up.j * m.FDPH[k];
+    bi = up.i * m.FDPH[k];
   }
 };
 
@@ -311,22 +304,53 @@ __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int 
 // form the float couplings of every plane and lane; otherwise the marching
 // wave forms them from the per-plane factors (less LDS per tile, so more
 // tiles resident per CU).
+//
+// Plane q of the tile lives in row q+1; a march step reads and writes row
+// clamp(q, -1, nk) + 1, so the steps outside the tile's planes land in row
+// 0 or row nk+1 and one index serves every access of a step.  The row the
+// steps before a lane's first plane use (row 0 forward, row nk+1 backward)
+// holds the incoming k value of the tile (the plane below / above) as its
+// right-hand side, pivot 1 and zero couplings and faces: those steps then
+// reproduce exactly that value, which is what the lane's first plane takes
+// as its k neighbour.  The other end row is scratch.
 template <int TK, bool STAGED>
 struct TileLds {
-  // per plane and lane: plane q in row q+1; row 0 is scratch, read and
-  // written by lanes whose step falls outside the tile's planes (one row
-  // index serves every access of a step; 16-plane tiles stay under 10 KB,
-  // so 16 fit in a CU's LDS)
-  float v[TK + 1][64];  // r (forward) or y (backward), overwritten by the result
-  float d[TK + 1][64];  // dinv
+  float v[TK + 2][64];  // r (forward) or y (backward), overwritten by the result
+  float d[TK + 2][64];  // dinv
   // couplings in k, j, i: forward A(p, lower), backward A(p, upper)
-  float ck[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
-  float cj[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
-  float ci[STAGED ? TK + 1 : 1][STAGED ? 64 : 1];
-  double dph[STAGED ? 1 : TK + 1], dpx[STAGED ? 1 : TK + 1];  // DPH and DPL / DPU
-  float fi[TK + 1][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
-  float fj[TK + 1][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
+  float ck[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
+  float cj[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
+  float ci[STAGED ? TK + 2 : 1][STAGED ? 64 : 1];
+  float2 pf[STAGED ? 1 : TK + 2];  // (FDPH, FDPL) forward, (FDPH, FDPU) backward
+  float fi[TK + 2][TJ];  // i face: forward t(i-1) at ii==0, backward z(i+1) at ii==TI-1
+  float fj[TK + 2][TI];  // j face: forward t(j-1) at jj==0, backward z(j+1) at jj==TJ-1
 };
+
+// The end rows of a tile (rows 0 and nk+1, see TileLds): in the row the
+// march enters from, right-hand side t_in, pivot 1; the other row zero;
+// zero faces, couplings and plane factors in both.  Wave 0 (the marching
+// wave) writes them, lane by lane.
+template <int TK, bool STAGED>
+__device__ __forceinline__ void tile_end_rows(TileLds<TK, STAGED>& sh, int entry, int other,
+                                              int lane, float t_in) {
+  sh.v[entry][lane] = t_in;
+  sh.d[entry][lane] = 1.f;
+  sh.v[other][lane] = 0.f;
+  sh.d[other][lane] = 0.f;
+  if constexpr (STAGED) {
+    sh.ck[entry][lane] = sh.cj[entry][lane] = sh.ci[entry][lane] = 0.f;
+    sh.ck[other][lane] = sh.cj[other][lane] = sh.ci[other][lane] = 0.f;
+  } else {
+    if (lane == 0) {
+      sh.pf[entry] = make_float2(0.f, 0.f);
+      sh.pf[other] = make_float2(0.f, 0.f);
+    }
+  }
+  if (lane < 8) {
+    sh.fi[entry][lane] = sh.fj[entry][lane] = 0.f;
+    sh.fi[other][lane] = sh.fj[other][lane] = 0.f;
+  }
+}
 
 // r, z, dinv and the FP32 sweep vector are all tile-major (TileMap): a
 // tile reads and writes whole 64-point planes.
@@ -365,9 +389,7 @@ __global__ void __launch_bounds__(64 * W)
   // incoming face, or the plane below the tile, is zeroed after loading).
   const bool lo_k0 = da.bl.lo_k(k0);
   const bool wrap = da.bl.wrap != 0;
-  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-  const double P1 = DRH * m.CP[jc];
-  const double dpu_last = m.DPU[m.c - 1];
+  const SweepCoup cl = sweep_coup_lower(m, ic, jc);
   // this tile's block of the tile-major FP32 arrays (plane q at tb + 64 q)
   constexpr size_t kTile = (size_t)TK * 64;
   const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
@@ -391,11 +413,11 @@ __global__ void __launch_bounds__(64 * W)
       fjv[e] = da.face_j[fbj + f];
     }
   }
-  double dphv = 0.0, dpxv = 0.0;
+  float dphv = 0.f, dpxv = 0.f;
   if constexpr (!kStaged) {
     const int kq = min(k0 + (int)threadIdx.x, g.c - 1);
-    dphv = m.DPH[kq];
-    dpxv = m.DPL[kq];
+    dphv = m.FDPH[kq];
+    dpxv = m.FDPL[kq];
   }
   // plane k0-1 (the last plane of the tile below) and plane 0 (seam)
   const size_t qo = tile_id(da, I, J, max(Kc - 1, 0)) * kTile + (TK - 1) * 64 + lane;
@@ -419,14 +441,16 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
       const int q = q0 + u;
-      sh.v[q + 1][lane] = rv[u];
-      sh.d[q + 1][lane] = dv[u];
+      if (q < nk) {  // rows past nk are not read (row nk+1 is an end row)
+        sh.v[q + 1][lane] = rv[u];
+        sh.d[q + 1][lane] = dv[u];
+      }
       if constexpr (kStaged) {
         if (q < nk) {
-          const double dph = m.DPH[k0 + q];
-          sh.ck[q + 1][lane] = (float)(-(P1 * m.DPL[k0 + q]));
-          sh.cj[q + 1][lane] = (float)(-((DRH * dph) * CTL));
-          sh.ci[q + 1][lane] = (float)(-((SD * dph) * CRL));
+          const float dph = m.FDPH[k0 + q];
+          sh.ck[q + 1][lane] = cl.k * m.FDPL[k0 + q];
+          sh.cj[q + 1][lane] = cl.j * dph;
+          sh.ci[q + 1][lane] = cl.i * dph;
         }
       }
     }
@@ -436,17 +460,16 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
   for (int e = 0; e < kFPT; e++) {
     const int f = (int)threadIdx.x + e * 64 * W;
-    if (f < TK * 8) {
+    if (f < nk * 8) {
       sh.fi[(f >> 3) + 1][f & 7] = face_i ? fiv[e] : 0.f;
       sh.fj[(f >> 3) + 1][f & 7] = face_j ? fjv[e] : 0.f;
     }
   }
   if constexpr (!kStaged) {
-    if ((int)threadIdx.x < TK) {
-      sh.dph[threadIdx.x + 1] = dphv;
-      sh.dpx[threadIdx.x + 1] = dpxv;
-    }
+    if ((int)threadIdx.x < nk) sh.pf[threadIdx.x + 1] = make_float2(dphv, dpxv);
   }
+  // the march enters every lane through row 0 with the plane below
+  if (wave == 0) tile_end_rows(sh, 0, nk + 1, lane, t_own);
   __syncthreads();
   if (wave == 0) {
     // Lane (ii, jj) handles plane q = s - ii - jj at step s.  t_k carries
@@ -454,22 +477,21 @@ __global__ void __launch_bounds__(64 * W)
     // what its i+1 / j+1 neighbours read one step later); it is refreshed
     // on every step, valid or not, because a neighbour only reads it on
     // steps where both planes are inside the tile.  The plane below the
-    // tile enters at q == 0 as t_init.  Only tiles holding plane c-1 of a
-    // wrapped phi range take the seam coupling (to plane 0).
+    // tile enters through row 0 (the steps before a lane's plane 0 reproduce
+    // it).  Only tiles holding plane c-1 of a wrapped phi range take the
+    // seam coupling (to plane 0).
     const bool wrap_tile = wrap && k1 == g.c;
-    const float aw = wrap_tile ? (float)(-(P1 * dpu_last)) : 0.f;
-    const float t_init = t_own;
+    const float aw = wrap_tile ? cl.k * m.FDPU[m.c - 1] : 0.f;
     const int S = nk + SKEW;
     auto march = [&](auto seam) {
       constexpr bool kSeam = decltype(seam)::value;
-      float t_k = t_init, tw = t_w;
+      float t_k = t_own, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float rr[8], dd[8], fl[8], fd[8], ak[8], aj[8], ai[8];
         int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qf = s0 + u - ii - jj;
-          const int qs = (unsigned)qf < (unsigned)nk ? qf + 1 : 0;
+          const int qs = min(max(s0 + u - ii - jj, -1), nk) + 1;
           rw[u] = qs;
           rr[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
@@ -478,10 +500,10 @@ __global__ void __launch_bounds__(64 * W)
             aj[u] = sh.cj[qs][lane];
             ai[u] = sh.ci[qs][lane];
           } else {
-            const double dph = sh.dph[qs];
-            ak[u] = (float)(-(P1 * sh.dpx[qs]));
-            aj[u] = (float)(-((DRH * dph) * CTL));
-            ai[u] = (float)(-((SD * dph) * CRL));
+            const float2 pf = sh.pf[qs];
+            ak[u] = cl.k * pf.y;
+            aj[u] = cl.j * pf.x;
+            ai[u] = cl.i * pf.x;
           }
           fl[u] = sh.fi[qs][jj];
           fd[u] = sh.fj[qs][ii];
@@ -495,8 +517,9 @@ __global__ void __launch_bounds__(64 * W)
           if constexpr (kSeam) {
             if (q == nk - 1) y = y - aw * tw;
           }
-          const float tk = (q > 0) ? t_k : t_init;
-          y = y - ak[u] * tk;
+          // the k neighbour: the lane's previous step (the end row's value
+          // before its first plane)
+          y = y - ak[u] * t_k;
           y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
           y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
           sh.v[rw[u]][lane] = y;
@@ -549,9 +572,7 @@ __global__ void __launch_bounds__(64 * W)
   // every coupling applied, as in the forward sweep; the upper neighbours
   // outside the grid are padding lanes, whose dinv (hence z) is zero
   const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
-  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-  const double P1 = DRH * m.CP[jc];
-  const double dpl_first = m.DPL[0];
+  const SweepCoup cu = sweep_coup_upper(m, ic, jc);
   constexpr size_t kTile = (size_t)TK * 64;
   const size_t tb = tile_id(da, I, J, Kc) * kTile + lane;
   // one batch of loads, as in the forward sweep: incoming faces (z of the
@@ -572,11 +593,11 @@ __global__ void __launch_bounds__(64 * W)
       fjv[e] = da.face_j[fbj + f];
     }
   }
-  double dphv = 0.0, dpxv = 0.0;
+  float dphv = 0.f, dpxv = 0.f;
   if constexpr (!kStaged) {
     const int kq = min(k0 + (int)threadIdx.x, g.c - 1);
-    dphv = m.DPH[kq];
-    dpxv = m.DPU[kq];
+    dphv = m.FDPH[kq];
+    dpxv = m.FDPU[kq];
   }
   const float fk = da.face_k[tile_id(da, I, J, min(Kc + 1, da.NK - 1)) * 64 + lane];
   const float fkt = da.face_kt[((size_t)I + (size_t)da.NI * J) * 64 + lane];
@@ -593,16 +614,18 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
     for (int u = 0; u < kRound; u++) {
       const int q = q0 + u;
-      // y outside the grid or past plane nk-1 is never written: zero it
-      const bool in = act && q < nk;
-      sh.v[q + 1][lane] = in ? yv8[u] : 0.f;
-      sh.d[q + 1][lane] = in ? dv[u] : 0.f;
+      // y outside the grid is never written: zero it (rows past nk are not
+      // read; row nk+1 is an end row)
+      if (q < nk) {
+        sh.v[q + 1][lane] = act ? yv8[u] : 0.f;
+        sh.d[q + 1][lane] = act ? dv[u] : 0.f;
+      }
       if constexpr (kStaged) {
         if (q < nk) {
-          const double dph = m.DPH[k0 + q];
-          sh.ck[q + 1][lane] = (float)(-(P1 * m.DPU[k0 + q]));
-          sh.cj[q + 1][lane] = (float)(-((DRH * dph) * CTH));
-          sh.ci[q + 1][lane] = (float)(-((SD * dph) * CRH));
+          const float dph = m.FDPH[k0 + q];
+          sh.ck[q + 1][lane] = cu.k * m.FDPU[k0 + q];
+          sh.cj[q + 1][lane] = cu.j * dph;
+          sh.ci[q + 1][lane] = cu.i * dph;
         }
       }
     }
@@ -612,36 +635,33 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
   for (int e = 0; e < kFPT; e++) {
     const int f = (int)threadIdx.x + e * 64 * W;
-    if (f < TK * 8) {
+    if (f < nk * 8) {
       sh.fi[(f >> 3) + 1][f & 7] = face_i ? fiv[e] : 0.f;
       sh.fj[(f >> 3) + 1][f & 7] = face_j ? fjv[e] : 0.f;
     }
   }
   if constexpr (!kStaged) {
-    if ((int)threadIdx.x < TK) {
-      sh.dph[threadIdx.x + 1] = dphv;
-      sh.dpx[threadIdx.x + 1] = dpxv;
-    }
+    if ((int)threadIdx.x < nk) sh.pf[threadIdx.x + 1] = make_float2(dphv, dpxv);
   }
+  // the march enters every lane through row nk+1 with the plane above
+  if (wave == 0) tile_end_rows(sh, nk + 1, 0, lane, t_own);
   __syncthreads();
   if (wave == 0) {
     // as in the forward sweep, mirrored: q = (nk-1) - (s - rii - rjj), the
-    // plane above the tile enters at q == nk-1, and only tiles holding
+    // plane above the tile enters through row nk+1, and only tiles holding
     // plane 0 of a wrapped phi range take the seam coupling (to c-1)
     const bool wrap_tile = wrap && k0 == 0;
-    const float bw = wrap_tile ? (float)(-(P1 * dpl_first)) : 0.f;
-    const float t_init = t_own;
+    const float bw = wrap_tile ? cu.k * m.FDPL[0] : 0.f;
     const int S = nk + SKEW;
     auto march = [&](auto seam) {
       constexpr bool kSeam = decltype(seam)::value;
-      float t_k = t_init, tw = t_w;
+      float t_k = t_own, tw = t_w;
       for (int s0 = 0; s0 < S; s0 += 8) {
         float yy[8], dd[8], fr[8], fu[8], bk[8], bj[8], bi[8];
         int rw[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int qf = (nk - 1) - (s0 + u - rii - rjj);
-          const int qs = (unsigned)qf < (unsigned)nk ? qf + 1 : 0;
+          const int qs = min(max((nk - 1) - (s0 + u - rii - rjj), -1), nk) + 1;
           rw[u] = qs;
           yy[u] = sh.v[qs][lane];
           dd[u] = sh.d[qs][lane];
@@ -650,10 +670,10 @@ __global__ void __launch_bounds__(64 * W)
             bj[u] = sh.cj[qs][lane];
             bi[u] = sh.ci[qs][lane];
           } else {
-            const double dph = sh.dph[qs];
-            bk[u] = (float)(-(P1 * sh.dpx[qs]));
-            bj[u] = (float)(-((DRH * dph) * CTH));
-            bi[u] = (float)(-((SD * dph) * CRH));
+            const float2 pf = sh.pf[qs];
+            bk[u] = cu.k * pf.y;
+            bj[u] = cu.j * pf.x;
+            bi[u] = cu.i * pf.x;
           }
           fr[u] = sh.fi[qs][jj];
           fu[u] = sh.fj[qs][ii];
@@ -666,8 +686,9 @@ __global__ void __launch_bounds__(64 * W)
           float sv = yy[u];
           sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
           sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-          const float tk = (q < nk - 1) ? t_k : t_init;
-          sv = sv - bk[u] * tk;
+          // the k neighbour: the lane's previous step (the end row's value
+          // before its first plane)
+          sv = sv - bk[u] * t_k;
           if constexpr (kSeam) {
             if (q == 0) sv = sv - bw * tw;
           }
@@ -747,7 +768,7 @@ struct TileLdsW {
   float fj[TK][8 * SI];        // incoming j face of the tile
   float xi[SI > 1 ? SI - 1 : 1][SJ][TK][8];  // i faces between waves
   float xj[SI][SJ > 1 ? SJ - 1 : 1][TK][8];  // j faces between waves
-  double dph[TK], dpx[TK];
+  float dph[TK], dpx[TK];  // FDPH and FDPL / FDPU
 };
 
 template <typename T, int SI, int SJ, int TK>
@@ -768,19 +789,17 @@ __global__ void __launch_bounds__(64 * SI * SJ)
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
   const bool lo_i = da.bl.lo_i(i), lo_j = da.bl.lo_j(j), lo_k0 = da.bl.lo_k(k0);
   const bool wrap = da.bl.wrap != 0;
-  const double DRH = m.DRH[ic], CRL = m.CRL[ic], SD = m.SD[jc], CTL = m.CTL[jc];
-  const double P1 = DRH * m.CP[jc];
-  const double dpu_last = m.DPU[m.c - 1];
+  const SweepCoup cl = sweep_coup_lower(m, ic, jc);
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (act && lo_k0) {
     const uint32_t q = col + g.ab * (uint32_t)(k0 - 1);
     t_own = dinv[q] * yv[q];
   }
   if (act && wrap && k0 > 0 && k1 == g.c) t_w = dinv[col] * yv[col];
-  double dph_s = 0.0, dpx_s = 0.0;
+  float dph_s = 0.f, dpx_s = 0.f;
   if (tid < nk) {
-    dph_s = m.DPH[k0 + tid];
-    dpx_s = m.DPL[k0 + tid];
+    dph_s = m.FDPH[k0 + tid];
+    dpx_s = m.FDPL[k0 + tid];
   }
   const bool face_i = (si == 0 && ii == 0 && lo_i), face_j = (sj == 0 && jj == 0 && lo_j);
   for (int q0 = 0; q0 < TK; q0 += 16) {
@@ -814,7 +833,7 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     sh.dpx[tid] = dpx_s;
   }
   __syncthreads();
-  const float aw = (wrap && k1 == g.c) ? (float)(-(P1 * dpu_last)) : 0.f;
+  const float aw = (wrap && k1 == g.c) ? cl.k * m.FDPU[m.c - 1] : 0.f;
   const int off = 8 * (si + sj);
   const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
   for (int T0 = 0; T0 < S; T0 += 8) {
@@ -826,10 +845,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
       if (act && q >= 0 && q < nk) {
         rr[u] = sh.v[q][tid];
         dd[u] = sh.d[q][tid];
-        const double dph = sh.dph[q];
-        ak[u] = (float)(-(P1 * sh.dpx[q]));
-        aj[u] = (float)(-((DRH * dph) * CTL));
-        ai[u] = (float)(-((SD * dph) * CRL));
+        const float dph = sh.dph[q];
+        ak[u] = cl.k * sh.dpx[q];
+        aj[u] = cl.j * dph;
+        ai[u] = cl.i * dph;
       }
     }
 #pragma unroll
@@ -881,16 +900,14 @@ __global__ void __launch_bounds__(64 * SI * SJ)
   const uint32_t col = (uint32_t)ic + (uint32_t)g.a * (uint32_t)jc;
   const bool hi_i = da.bl.hi_i(i, g.a), hi_j = da.bl.hi_j(j, g.b);
   const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = da.bl.wrap != 0;
-  const double DRH = m.DRH[ic], CRH = m.CRH[ic], SD = m.SD[jc], CTH = m.CTH[jc];
-  const double P1 = DRH * m.CP[jc];
-  const double dpl_first = m.DPL[0];
+  const SweepCoup cu = sweep_coup_upper(m, ic, jc);
   float t_own = 0.f, t_w = 0.f, t_pub = 0.f;
   if (act && hi_k1) t_own = yv[col + g.ab * (uint32_t)k1];
   if (act && wrap && k1 < g.c && k0 == 0) t_w = yv[col + g.ab * (uint32_t)(g.c - 1)];
-  double dph_s = 0.0, dpx_s = 0.0;
+  float dph_s = 0.f, dpx_s = 0.f;
   if (tid < nk) {
-    dph_s = m.DPH[k0 + tid];
-    dpx_s = m.DPU[k0 + tid];
+    dph_s = m.FDPH[k0 + tid];
+    dpx_s = m.FDPU[k0 + tid];
   }
   const bool face_i = (si == SI - 1 && ii == 7 && hi_i);
   const bool face_j = (sj == SJ - 1 && jj == 7 && hi_j);
@@ -925,7 +942,7 @@ __global__ void __launch_bounds__(64 * SI * SJ)
     sh.dpx[tid] = dpx_s;
   }
   __syncthreads();
-  const float bw = (wrap && k0 == 0) ? (float)(-(P1 * dpl_first)) : 0.f;
+  const float bw = (wrap && k0 == 0) ? cu.k * m.FDPL[0] : 0.f;
   const int off = 8 * ((SI - 1 - si) + (SJ - 1 - sj));
   const int S = nk + SKEW + 8 * (SI - 1) + 8 * (SJ - 1);
   for (int T0 = 0; T0 < S; T0 += 8) {
@@ -937,10 +954,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
       if (act && q >= 0 && q < nk) {
         yy[u] = sh.v[q][tid];
         dd[u] = sh.d[q][tid];
-        const double dph = sh.dph[q];
-        bk[u] = (float)(-(P1 * sh.dpx[q]));
-        bj[u] = (float)(-((DRH * dph) * CTH));
-        bi[u] = (float)(-((SD * dph) * CRH));
+        const float dph = sh.dph[q];
+        bk[u] = cu.k * sh.dpx[q];
+        bj[u] = cu.j * dph;
+        bi[u] = cu.i * dph;
       }
     }
 #pragma unroll

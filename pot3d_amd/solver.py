@@ -1,8 +1,9 @@
 """Host-side mirror of POT3D's solve path over libpot3d_hip.so.
 
-Names follow the reference (src/pot3d.F90): ``potfld`` (:3866-3976),
-``solve`` (:4169-4229, returning ierr), ``ax`` (:5102-5150), ``prec_inv``
-(:5152-5225), ``getb`` / ``magnetic_energy`` (:6187-6321).  Vectors are
+Names follow the reference (src/pot3d.F90): ``readbr`` (:6751-6829, with
+set_flux :3806-3833), ``potfld`` (:3866-3976), ``solve`` (:4169-4229,
+returning ierr), ``ax`` (:5102-5150), ``prec_inv`` (:5152-5225), ``getb`` /
+``magnetic_energy`` (:6187-6321).  Vectors are
 numpy float64 arrays in the reference's packed layout (N=(nr-2)(nt-2)(np-2),
 r fastest); phi is the padded (nr,nt,np) Fortran-ordered array, returned
 flat.  Every call runs on the GPU through the C ABI; there is no CPU path.
@@ -27,6 +28,29 @@ def _ptr(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def readbr(mesh, tn, pn, bn, balance_flux: bool = True, abs_br: bool = False,
+           pl: float = 2.0 * 3.1415926535897932 - 0.0):
+    """The driver's readbr after the HDF5 read, and set_flux, on the GPU
+    (p3d_readbr): the map bn(len(tn), len(pn)) on its theta/phi scales
+    interpolated onto the mesh, seam, m=0 poles, flux balance (balance_flux)
+    and |Br| (abs_br, option 'open').  Returns (br0 as an (nt, np) C array
+    holding the Fortran br0(j,k) at [j-1, k-1], dict(fluxp, fluxm, area,
+    br00err))."""
+    tn = _f64(tn)
+    pn = _f64(pn)
+    bnf = np.asfortranarray(bn, dtype=np.float64)
+    assert bnf.shape == (tn.size, pn.size)
+    nt, np_ = mesh.nt, mesh.np
+    br0 = np.empty((np_, nt))  # Fortran (nt, np) = C (np, nt)
+    flux = np.zeros(4)
+    arrs = [_f64(a) for a in (mesh.th, mesh.ph, mesh.sth, mesh.dth, mesh.dph)]
+    check(_lib.load().p3d_readbr(nt, np_, *(_ptr(a) for a in arrs), float(pl), tn.size, pn.size,
+                                 _ptr(tn), _ptr(pn), bnf.ctypes.data_as(_dp), int(balance_flux),
+                                 int(abs_br), _ptr(br0), _ptr(flux)))
+    return np.ascontiguousarray(br0.T), dict(zip(("fluxp", "fluxm", "area", "br00err"),
+                                                 map(float, flux)))
 
 
 class Pot3dSolver:

@@ -97,6 +97,31 @@ class WarmStartNet:
                                        C.byref(loss), g.ctypes.data_as(_fp) if grads else None))
         return loss.value, g
 
+    def set_augment(self, phi_roll: bool, theta_flip: bool, sign_flip: bool):
+        check(self._L.p3d_nn_set_augment(self._h, int(phi_roll), int(theta_flip),
+                                         int(sign_flip)))
+
+    def save(self, path: str, **meta):
+        """Parameters, scales and the grid (plus ``meta``) in an .npz file."""
+        np.savez_compressed(path, params=self.params(), bscale=self.bscale,
+                            phiscale=self.phiscale, grid=np.array([self.nr, self.nt, self.np]),
+                            **meta)
+
+    @classmethod
+    def load(cls, path: str, mesh):
+        """A network for ``mesh`` with the parameters and scales of ``path``
+        (written by save); returns (net, the file's other entries)."""
+        z = np.load(path, allow_pickle=False)
+        grid = tuple(int(v) for v in z["grid"])
+        if grid != (mesh.nr, mesh.nt, mesh.np):
+            raise ValueError(f"{path} holds a network for {grid}, not "
+                             f"{(mesh.nr, mesh.nt, mesh.np)}")
+        net = cls(mesh, seed=0)
+        net.set_params(z["params"])
+        net.set_scales(float(z["bscale"]), float(z["phiscale"]))
+        meta = {k: z[k] for k in z.files if k not in ("params", "bscale", "phiscale", "grid")}
+        return net, meta
+
     def train(self, nsteps: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         losses = np.zeros(max(nsteps, 1), np.float32)
         check(self._L.p3d_nn_train(self._h, int(nsteps), float(lr), int(seed) & 0xFFFFFFFF,

@@ -30,16 +30,25 @@ def oracle_mod():
     return O
 
 
+# The synthetic workloads in the tests take br0 from the oracle's readbr
+# (the CPU restatement); the GPU's p3d_readbr is checked against it by
+# tests/test_gpu_readbr.py.
+import readbr as _oracle_readbr  # noqa: E402
+from p3d_driver import workloads as _workloads  # noqa: E402
+
+_workloads.set_readbr(lambda mesh, tn, pn, bn, balance, abs_br:
+                      _oracle_readbr.readbr(mesh, tn, pn, bn, balance, abs_br)[0])
+
+
 def small_mesh(nr=9, nt=8, np_=12, drratio=1.5, r0=1.0, r1=2.5):
-    from pot3d_amd import inputs
+    from p3d_driver import inputs
     return inputs.global_mesh(nr, nt, np_, r0, r1, rfrac=(0.0, 1.0), drratio=(drratio,))
 
 
 def random_br0(mesh, seed=0):
     """A smooth-ish random br0(nt,np) with the driver's periodic/pole
     treatment and flux balance applied (readbr's post-processing)."""
-    from pot3d_amd import inputs
     rng = np.random.default_rng(seed)
     br0 = np.zeros((mesh.nt, mesh.np))
     br0[1:-1, 1:-1] = rng.standard_normal((mesh.nt - 2, mesh.np - 2))
-    return inputs._finish_br0(mesh, br0, "ss", False)
+    return _oracle_readbr.finish(mesh, br0, True, False)[0]

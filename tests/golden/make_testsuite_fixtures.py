@@ -11,8 +11,10 @@ Writes, next to this script:
                                  as float64 arrays in (theta, phi) order;
   * testsuite_reference.json  -- per deck: the namelist values the driver
                                  needs, the energies (ES22.16), the CG
-                                 iteration count, the residual history, and
-                                 the printed r/t/p mesh tables.
+                                 iteration count, the residual history, the
+                                 printed r/t/p mesh tables, and readbr's
+                                 printed flux balance (positive and negative
+                                 flux, BR00 when the flux was balanced).
 Only data is copied: inputs and the outputs the reference recorded.
 """
 import json
@@ -72,6 +74,16 @@ def parse_out(text):
     return res
 
 
+def parse_log(text):
+    """readbr's flux prints (pot3d.log): Positive/Negative flux and, when
+    the flux was balanced, BR00."""
+    res = {"fluxp": float(re.search(r"Positive flux =\s*(\S+)", text).group(1)),
+           "fluxm": float(re.search(r"Negative flux =\s*(\S+)", text).group(1))}
+    m = re.search(r"BR00 \(monopole Br field magnitude\) =\s*(\S+)", text)
+    res["br00err"] = float(m.group(1)) if m else None
+    return res
+
+
 def main(ref):
     here = os.path.dirname(os.path.abspath(__file__))
     out = {}
@@ -85,6 +97,7 @@ def main(ref):
         np.savez_compressed(os.path.join(here, f"testsuite_br_{deck}.npz"), data=data, tn=tn, pn=pn)
         rec = parse_out(open(os.path.join(base, "reference", "pot3d.out")).read())
         rec["namelist"] = nl
+        rec["readbr"] = parse_log(open(os.path.join(base, "reference", "pot3d.log")).read())
         out[deck] = rec
     with open(os.path.join(here, "testsuite_reference.json"), "w") as f:
         json.dump(out, f, indent=1)

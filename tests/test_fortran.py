@@ -25,7 +25,7 @@ def test_fortran_module_compiles_and_links():
 
 def _demo_mesh():
     """The uniform meshes p3d_fortran_demo.f90 builds (nr=17, nt=19, np=28)."""
-    from pot3d_amd import inputs
+    from p3d_driver import inputs
     nr, nt, np_ = 17, 19, 28
     pi = inputs.PI
     r = np.array([1.0 + 1.5 * (i - 1) / (nr - 2) for i in range(1, nr)])

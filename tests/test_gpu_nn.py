@@ -107,7 +107,7 @@ def test_forward_and_gradient_match_oracle(dims):
 
 def test_prolongation_exact_for_multilinear():
     from pot3d_amd import warmstart as W
-    from pot3d_amd.inputs import global_mesh
+    from p3d_driver.inputs import global_mesh
     say("[gpu] trilinear prolongation")
     cm = global_mesh(9, 10, 14, 1.0, 2.5, rfrac=(0.0, 1.0), drratio=(2.0,))
     fm = global_mesh(17, 19, 27, 1.0, 2.5, rfrac=(0.0, 1.0), drratio=(2.0,))
@@ -134,7 +134,7 @@ def _pairs(wl_fn, seeds, solver):
 def test_training_lowers_loss_and_warm_start_gains():
     from pot3d_amd import _lib
     from pot3d_amd import warmstart as W
-    from pot3d_amd import workloads
+    from p3d_driver import workloads
     from pot3d_amd.solver import Pot3dSolver
     say("[gpu] train on PCG pairs, warm start on held-out maps")
 

@@ -55,6 +55,39 @@ def test_operator(oracle_mod, solver_mod, name, pm_r1):
             assert np.max(np.abs(y - ref) / scale) < 1e-14
 
 
+# grids for the operator kernels' layouts: a < 62 and > 62 (several waves
+# along i), b not a multiple of 8 (a partial row block), few phi planes
+# (lookahead past the segment wraps more than once)
+KERNEL_GRIDS = [(9, 8, 12), (13, 11, 7), (33, 45, 70), (70, 27, 40), (140, 19, 23),
+                (131, 67, 9)]
+
+
+@pytest.mark.parametrize("dims", KERNEL_GRIDS)
+@pytest.mark.parametrize("pm_r1", [-1.0, 1.0])
+def test_operator_kernels_agree(oracle_mod, solver_mod, dims, pm_r1):
+    """The row-blocked operator kernel (P3D_OPT_STENCIL=1) gives the column
+    march's values bit for bit, in FP64 and in the FP32 operator of the
+    mixed recurrence; the FP64 values against the oracle as test_operator."""
+    from pot3d_amd import _lib
+    say(f"[gpu] operator kernels {dims} pm_r1={pm_r1}")
+    nr, nt, np_ = dims
+    mesh = small_mesh(nr=nr, nt=nt, np_=np_, drratio=1.7)
+    o = oracle_mod.Oracle(mesh, pm_r1)
+    with solver_mod.Pot3dSolver(mesh, pm_r1, 1) as s:
+        x = np.random.default_rng(11).standard_normal(o.N)
+        for fp32 in (0, 1):
+            s.set_option(_lib.OPT_APPLY_FP32, fp32)
+            s.set_option(_lib.OPT_STENCIL, 0)
+            y0 = s.ax(x)
+            s.set_option(_lib.OPT_STENCIL, 1)
+            y1 = s.ax(x)
+            assert np.array_equal(y0.view(np.uint64), y1.view(np.uint64)), fp32
+            ref = o.ax(x)
+            scale = np.abs(o.coefs()).sum(axis=0) * np.abs(x).max()
+            tol = 1e-14 if not fp32 else 1e-6
+            assert np.max(np.abs(y1 - ref) / scale) < tol
+
+
 @pytest.mark.parametrize("name", ["tiny", "mid"])
 def test_rhs_phi_energy(oracle_mod, solver_mod, name):
     say(f"[gpu] rhs/phi/energy mesh={name}")
@@ -324,7 +357,8 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
     iteration count and three residual replacements for 1e-9, and lands on
     the FP64 potential and energy."""
     say("[gpu] mixed mode 2 on the open problem")
-    from pot3d_amd import _lib, workloads
+    from pot3d_amd import _lib
+    from p3d_driver import workloads
     w = workloads.pfss(41, 81, 160, seed=3, n_bipoles=16, option="open", r0=2.5, r1=10.0,
                        ifprec=2)
     with solver_mod.Pot3dSolver(w.mesh, w.pm_r1, 2) as s:

@@ -19,7 +19,8 @@ DECKS = ["potential_field_source_surface", "potential_field_current_sheet", "ope
 def test_deck(deck, ifprec):
     print(f"[gpu] testsuite deck={deck} ifprec={ifprec}", flush=True)
     sys.stdout.flush()
-    from pot3d_amd import solver, workloads
+    from pot3d_amd import solver
+    from p3d_driver import workloads
     w = workloads.testsuite_deck(deck)
     ref = workloads.testsuite_reference(deck)
     res = solver.potfld(w, ifprec=ifprec, ncgmax=3 * ref["iterations"])

@@ -1,12 +1,12 @@
 """Driver-side inputs: genmesh/set_global_mesh against the mesh tables the
 reference printed for its three test decks (8 decimals,
-testsuite/*/reference/pot3d.out), and readbr's post-processing."""
+testsuite/*/reference/pot3d.out), and the seeded synthetic Br maps."""
 import math
 
 import numpy as np
 import pytest
 
-from pot3d_amd import inputs, workloads
+from p3d_driver import inputs, workloads
 
 DECKS = ["potential_field_source_surface", "potential_field_current_sheet", "open_field"]
 
@@ -30,26 +30,11 @@ def test_half_mesh_relations():
     assert m.dp[0] == m.dp[-2] and m.dph[-1] == m.dph[1]
 
 
-def test_readbr_pole_periodic_and_flux_balance():
-    m = inputs.global_mesh(10, 31, 61)
-    tn = np.linspace(0, math.pi, 46)
-    pn = np.linspace(0, 2 * math.pi, 91)
-    T, P = np.meshgrid(tn, pn, indexing="ij")
-    bn = 3.0 * np.cos(T) + np.sin(T) * np.cos(P) + 0.7
-    br0 = inputs.readbr_interp(m, tn, pn, bn, option="ss")
-    assert np.array_equal(br0[:, 0], br0[:, -2]) and np.array_equal(br0[:, -1], br0[:, 1])
-    da = (m.sth[1:-1, None] * m.dth[1:-1, None]) * m.dph[None, 1:-1]
-    assert abs(np.sum(br0[1:-1, 1:-1] * da)) < 1e-12 * np.sum(np.abs(br0[1:-1, 1:-1]) * da)
-    w = m.dph[1:-1] / (2 * math.pi)
-    s0 = np.sum(br0[1, 1:-1] * w)
-    assert np.allclose(br0[0, :], 2 * s0 - br0[1, :])
-
-
 def test_synthetic_br_is_seeded_and_balanced():
     m = inputs.global_mesh(10, 46, 91)
-    a = inputs.synthetic_br0(m, 3)
-    b = inputs.synthetic_br0(m, 3)
-    c = inputs.synthetic_br0(m, 4)
+    a = workloads.synthetic_br0(m, 3)
+    b = workloads.synthetic_br0(m, 3)
+    c = workloads.synthetic_br0(m, 4)
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     assert np.abs(a).max() > 40.0
 

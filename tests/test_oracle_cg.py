@@ -42,7 +42,7 @@ def test_ncgmax_gives_ierr1(oracle_mod):
 def test_energy_of_uniform_field(oracle_mod):
     """phi = r cos(theta) gives B = grad phi with |B| = 1, so the energy is
     half the shell volume (2 pi/3)(r1^3 - r0^3), to O(h^2)."""
-    from pot3d_amd import inputs
+    from p3d_driver import inputs
 
     def rel_err(nr, nt, np_):
         mesh = inputs.global_mesh(nr, nt, np_, 1.0, 2.0, rfrac=(0.0, 1.0), drratio=(1.0,))
@@ -85,7 +85,7 @@ def test_testsuite_deck_energies(oracle_mod, deck):
     """The reference's own pass rule: energies equal to 7 decimal places
     (bin/pot3d_validation.sh), plus the recorded iteration count and
     residual history (testsuite/<deck>/reference/pot3d.out)."""
-    from pot3d_amd import workloads
+    from p3d_driver import workloads
     w = workloads.testsuite_deck(deck)
     ref = workloads.testsuite_reference(deck)
     o = oracle_mod.Oracle(w.mesh, w.pm_r1)
