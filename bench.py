@@ -375,7 +375,7 @@ def main():
             xc = net.predict(brc)
             phic = sc.fill_phi(xc)
             hip.h2d(d_phic, phic)
-            W.prolong_dev(tab, d_phic, dx)
+            W.prolong_dev_noalloc(tab, d_phic, dx)
 
         twl, stwl, xwl = timed(f"mixed(mode {args.mixed_mode}) + learned block-Jacobi ILU0 PCG",
                                args.mixed_mode, guess, warmup=args.warmup, steps=args.steps)

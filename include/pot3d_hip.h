@@ -299,6 +299,18 @@ int p3d_nn_prolong(int nrc, int ntc, int npc, const double* phic, int af, int bf
 int p3d_nn_prolong_dev(int nrc, int ntc, int npc, const double* phic_dev, int af, int bf, int cf,
                        const int* ir, const double* wr, const int* it, const double* wt,
                        const int* ip, const double* wp, double* xf_dev);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): p3d_nn_prolong_dev with the same arguments and results,
+ * but the per-axis tables are staged in a fixed device array of the
+ * library (96 KiB per GPU, held until the process ends) instead of buffers
+ * allocated per call; no hipMalloc or hipFree.  af+bf+cf must be at most
+ * 8192 (else P3D_EINVAL).  Calls that overlap on one GPU race on that
+ * array. */
+int p3d_nn_prolong_dev_noalloc(int nrc, int ntc, int npc, const double* phic_dev, int af, int bf,
+                               int cf, const int* ir, const double* wr, const int* it,
+                               const double* wt, const int* ip, const double* wp,
+                               double* xf_dev);
 /* One 16x16x16 BF16 matrix-core product D = A B (row-major BF16 bits in,
  * FP32 out), for checking the operand layout the network kernels use. */
 int p3d_nn_mfma_check(const unsigned short* A, const unsigned short* B, float* D);

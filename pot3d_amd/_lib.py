@@ -56,6 +56,8 @@ SIGNATURES = {
                                  _ip, _dp, _ip, _dp, _ip, _dp, _dp]),
     "p3d_nn_prolong_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                      _ip, _dp, _ip, _dp, _ip, _dp, _vp]),
+    "p3d_nn_prolong_dev_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+                                             C.c_int, _ip, _dp, _ip, _dp, _ip, _dp, _vp]),
     "p3d_nn_mfma_check": (C.c_int, [C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), _fp]),
 }
 

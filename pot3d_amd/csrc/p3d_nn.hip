@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -31,6 +32,40 @@ int set_error(int code, const char* msg);
 }
 
 static int nfail(int code, const char* msg) { return p3d::set_error(code, msg); }
+
+// Staging for p3d_nn_prolong_dev_noalloc: the per-axis tables of one
+// prolongation (weights wr|wt|wp as doubles, then indices ir|it|ip as ints),
+// in a file-scope device array, so that entry point allocates nothing.
+// Every GPU the library runs on holds its own copy (96 KiB) until the
+// process ends.  Host code never uses the array's name as an address: the
+// device address is looked up once per device (scratch_on_device) and that
+// pointer is what the copies and the kernel receive.
+constexpr int kProlongMaxTable = 8192;  // af + bf + cf at most
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kProlongMaxTable + kProlongMaxTable / 2];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up on the
+// first call with that device current), or nullptr after an error.
+static double* scratch_on_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    nfail(P3D_EHIP, "no current HIP device (or device id beyond 63)");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      nfail(P3D_EHIP, "hipGetSymbolAddress of the prolongation scratch failed");
+      return nullptr;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_dev[dev];
+}
 
 struct ConvSpec {
   int Cin, Cout, level;
@@ -520,6 +555,43 @@ int p3d_nn_prolong_dev(int nrc, int ntc, int npc, const double* phic_dev, int af
                        const int* ir, const double* wr, const int* it, const double* wt,
                        const int* ip, const double* wp, double* xf_dev) {
   return prolong_impl(nrc, ntc, npc, phic_dev, af, bf, cf, ir, wr, it, wt, ip, wp, xf_dev, true);
+}
+
+// p3d_nn_prolong_dev with its tables staged in the device array `scratch`
+// instead of buffers allocated per call: no hipMalloc/hipFree.  Rejects
+// af + bf + cf > kProlongMaxTable.  The tables are rewritten whole on every
+// call, so nothing needs clearing; two calls overlapping on one GPU race on
+// them.
+int p3d_nn_prolong_dev_noalloc(int nrc, int ntc, int npc, const double* phic_dev, int af, int bf,
+                               int cf, const int* ir, const double* wr, const int* it,
+                               const double* wt, const int* ip, const double* wp, double* xf_dev) {
+  const int rc =
+      prolong_check(nrc, ntc, npc, phic_dev, af, bf, cf, ir, wr, it, wt, ip, wp, xf_dev);
+  if (rc != P3D_OK) return rc;
+  if ((long long)af + bf + cf > kProlongMaxTable) {
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "prolongation tables of %lld entries (af+bf+cf) exceed the no-allocation limit "
+             "of %d",
+             (long long)af + bf + cf, kProlongMaxTable);
+    return nfail(P3D_EINVAL, msg);
+  }
+  double* ptr = scratch_on_device();
+  if (!ptr) return P3D_EHIP;
+  double* dw = ptr;                                      // wr | wt | wp
+  int* di = reinterpret_cast<int*>(ptr + kProlongMaxTable);  // ir | it | ip
+  bool ok = hipMemcpy(di, ir, (size_t)af * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(di + af, it, (size_t)bf * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(di + af + bf, ip, (size_t)cf * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dw, wr, (size_t)af * 8, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dw + af, wt, (size_t)bf * 8, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dw + af + bf, wp, (size_t)cf * 8, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) return nfail(P3D_EHIP, "table copy failed");
+  prolong(af, bf, cf, nrc, ntc, npc, phic_dev, di, dw, di + af, dw + af, di + af + bf,
+          dw + af + bf, xf_dev, 0);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return nfail(P3D_EHIP, "prolongation failed");
+  return P3D_OK;
 }
 
 // One 16x16x16 BF16 MFMA: D = A*B (A, B row-major BF16 bits, D FP32).

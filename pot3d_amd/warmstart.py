@@ -194,6 +194,14 @@ def prolong_dev(tab: ProlongTables, phic_ptr: int, xf_ptr: int):
                                          C.c_void_p(xf_ptr)))
 
 
+def prolong_dev_noalloc(tab: ProlongTables, phic_ptr: int, xf_ptr: int):
+    """prolong_dev through the entry point that allocates nothing (tables
+    staged in the library's fixed device array; af+bf+cf <= 8192)."""
+    check(_lib.load().p3d_nn_prolong_dev_noalloc(tab.nrc, tab.ntc, tab.npc, C.c_void_p(phic_ptr),
+                                                 tab.af, tab.bf, tab.cf, *tab.args(),
+                                                 C.c_void_p(xf_ptr)))
+
+
 # -- training data -------------------------------------------------------------
 def _rms_interior(br0):
     b = np.asarray(br0)[1:-1, 1:-1]
