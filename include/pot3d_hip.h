@@ -124,10 +124,12 @@ enum {
   P3D_OPT_TILE_STAGE = 14,  /* 8x8-column tiles: 0 planes/8 waves load each
                                 tile and one marches (default); 1 the
                                 marching wave loads its tile alone (one wave
-                                per workgroup, more tiles resident per CU).
-                                Same results bit for bit. */
-  P3D_OPT_STENCIL = 15,     /* operator kernel: 0 one thread per (i,j)
-                                column marching phi; 1 (default) each thread
+                                per workgroup, more tiles resident per CU);
+                                2 the marching wave holds its tile in
+                                registers (16 or 32 planes, one wave per
+                                tile).  Same results bit for bit. */
+  P3D_OPT_STENCIL = 15,     /* operator kernel: 0 (default) one thread per
+                                (i,j) column marching phi; 1 each thread
                                 holds 8 theta rows of one column, the
                                 neighbours in i from the adjacent lanes.
                                 Same operator values bit for bit. */

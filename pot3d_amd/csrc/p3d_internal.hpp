@@ -120,6 +120,14 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
 // diacsr drops the couplings at its local r and theta boundaries and, when
 // nproc_p > 1, the phi ones (src/pot3d.F90:4940-4967).  Extents >= (a,b,c):
 // no cut.
+// y - a*b rounded once: the update every D-ILU triangular-solve schedule
+// applies to a point, written as an explicit fused multiply-add so that the
+// rounding does not depend on where the compiler places the product (all
+// schedules must agree bit for bit).
+__device__ __forceinline__ float sub_prod(float y, float a, float b) {
+  return __builtin_fmaf(-a, b, y);
+}
+
 struct IluBlocks {
   int bi, bj, bk;
   int wrap;  // phi wrap kept

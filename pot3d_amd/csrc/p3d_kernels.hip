@@ -1136,11 +1136,11 @@ __global__ void k_fwd_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks
   float y = (float)r[n];
   if (wrap && k == g.c - 1 && g.c > 1) {
     const uint32_t q = (uint32_t)i + (uint32_t)g.a * (uint32_t)j;
-    y = y - (float)aw * (dinv[q] * yv[q]);
+    y = sub_prod(y, (float)aw, (dinv[q] * yv[q]));
   }
-  if (bl.lo_k(k)) y = y - (float)ak * (dinv[n - g.ab] * yv[n - g.ab]);
-  if (bl.lo_j(j)) y = y - (float)aj * (dinv[n - g.a] * yv[n - g.a]);
-  if (bl.lo_i(i)) y = y - (float)ai * (dinv[n - 1] * yv[n - 1]);
+  if (bl.lo_k(k)) y = sub_prod(y, (float)ak, (dinv[n - g.ab] * yv[n - g.ab]));
+  if (bl.lo_j(j)) y = sub_prod(y, (float)aj, (dinv[n - g.a] * yv[n - g.a]));
+  if (bl.lo_i(i)) y = sub_prod(y, (float)ai, (dinv[n - 1] * yv[n - 1]));
   yv[n] = y;
 }
 
@@ -1157,10 +1157,10 @@ __global__ void k_bwd_level(CP cp, Grid3 g, int L, int kmin, int wrap, IluBlocks
   double bi, bj, bk, bw;
   cp.upper(i, j, k, n, bi, bj, bk, bw);
   float s = yv[n];
-  if (bl.hi_i(i, g.a)) s = s - (float)bi * yv[n + 1];
-  if (bl.hi_j(j, g.b)) s = s - (float)bj * yv[n + g.a];
-  if (bl.hi_k(k, g.c)) s = s - (float)bk * yv[n + g.ab];
-  if (wrap && k == 0 && g.c > 1) s = s - (float)bw * yv[n + (uint32_t)(g.c - 1) * g.ab];
+  if (bl.hi_i(i, g.a)) s = sub_prod(s, (float)bi, yv[n + 1]);
+  if (bl.hi_j(j, g.b)) s = sub_prod(s, (float)bj, yv[n + g.a]);
+  if (bl.hi_k(k, g.c)) s = sub_prod(s, (float)bk, yv[n + g.ab]);
+  if (wrap && k == 0 && g.c > 1) s = sub_prod(s, (float)bw, yv[n + (uint32_t)(g.c - 1) * g.ab]);
   const float zz = s * dinv[n];
   yv[n] = zz;
   z[n] = (T)zz;

@@ -871,7 +871,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     s->blocks = IluBlocks{s->g.a, s->g.b, s->g.c, 1};
     if ((rc = factor_ilu(s))) return bail(rc);
     s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
-                                       &s->blocks, s->dinv, s->tile_stage) == 0);
+                                       &s->blocks, s->dinv, s->tile_stage, &s->m) == 0);
     if ((rc = update_tilemap(s))) return bail(rc);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -917,10 +917,12 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int prc =
       build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv,
-                       s->tile_stage);
+                       s->tile_stage, &s->m);
   s->sweep_ready = prc == 0;
   int rc = update_tilemap(s);
   if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
+  if (prc == 3)
+    return fail(P3D_EINVAL, "P3D_OPT_TILE_STAGE 2 needs one wave per tile and 16 or 32 planes");
   if (!s->sweep_ready) return fail(P3D_ENOMEM, "sweep plan allocation failed");
   return rc;
 }
@@ -984,8 +986,8 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
         return fail(P3D_EINVAL, "P3D_OPT_TILE_K must be 16, 32 or 64");
       if (key == P3D_OPT_TILE_WAVES && v != 1 && v != 4 && v != 8)
         return fail(P3D_EINVAL, "P3D_OPT_TILE_WAVES must be 1, 4 or 8");
-      if (key == P3D_OPT_TILE_STAGE && v != 0 && v != 1)
-        return fail(P3D_EINVAL, "P3D_OPT_TILE_STAGE must be 0 or 1");
+      if (key == P3D_OPT_TILE_STAGE && v != 0 && v != 1 && v != 2)
+        return fail(P3D_EINVAL, "P3D_OPT_TILE_STAGE must be 0, 1 or 2");
       if (key == P3D_OPT_TILE_K) s->tile_k = v;
       else if (key == P3D_OPT_TILE_WAVES) s->tile_waves = v;
       else s->tile_stage = v;

@@ -142,10 +142,10 @@ __device__ void fwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
       lanes_from_lower(t_pub, from_i, from_j);
       if (act && k >= k0 && k < k1) {
         float y = rr[u];
-        if (k == g.c - 1) y = y - aw * t_w;
-        if (k > 0) y = y - ak[u] * t_own;
-        if (j > 0) y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-        if (i > 0) y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+        if (k == g.c - 1) y = sub_prod(y, aw, t_w);
+        if (k > 0) y = sub_prod(y, ak[u], t_own);
+        if (j > 0) y = sub_prod(y, aj[u], (jj == 0 ? fd[u] : from_j));
+        if (i > 0) y = sub_prod(y, ai[u], (ii == 0 ? fl[u] : from_i));
         yv[col + g.ab * (uint32_t)k] = y;
         const float t = dd[u] * y;
         if (k == 0) t_w = t;
@@ -195,10 +195,10 @@ __device__ void bwd_tile(CS cs, const Grid3& g, int I, int J, int k0, int k1,
       lanes_from_upper(t_pub, from_i, from_j);
       if (act && k >= k0 && k < k1) {
         float sv = yy[u];
-        if (i < g.a - 1) sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-        if (j < g.b - 1) sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
-        if (k < g.c - 1) sv = sv - bk[u] * t_own;
-        if (k == 0) sv = sv - bw * t_w;
+        if (i < g.a - 1) sv = sub_prod(sv, bi[u], (ii == TI - 1 ? fr[u] : from_i));
+        if (j < g.b - 1) sv = sub_prod(sv, bj[u], (jj == TJ - 1 ? fu[u] : from_j));
+        if (k < g.c - 1) sv = sub_prod(sv, bk[u], t_own);
+        if (k == 0) sv = sub_prod(sv, bw, t_w);
         const float zz = sv * dd[u];
         const uint32_t p = col + g.ab * (uint32_t)k;
         yv[p] = zz;
@@ -262,6 +262,7 @@ struct DiagArgs {
   float* yv_t;          // forward result y in FP32, tile-major
   float* face_k;        // per tile [64]: backward z of its first plane (for the tile below)
   float* face_kt;       // per column of tiles [64]: backward z of plane c-1 (phi seam)
+  int count;            // tiles on this diagonal
 };
 
 __device__ __forceinline__ void diag_tile(const DiagArgs& da, int b, int& I, int& J, int& Kc) {
@@ -293,7 +294,7 @@ static DiagArgs diag_args(const SweepPlan& plan, int L) {
   return DiagArgs{L,       plan.TNI, plan.TNJ,    plan.NK,
                   plan.blocked ? plan.tiles + plan.diag_off[L] : nullptr,
                   plan.bl, plan.face_i, plan.face_j, plan.dinv_t, plan.yv_t,
-                  plan.face_k, plan.face_kt};
+                  plan.face_k, plan.face_kt, plan.diag_off[L + 1] - plan.diag_off[L]};
 }
 
 __device__ __forceinline__ size_t tile_id(const DiagArgs& da, int I, int J, int Kc) {
@@ -515,13 +516,13 @@ __global__ void __launch_bounds__(64 * W)
           lanes_from_lower(t_k, from_i, from_j);
           float y = rr[u];
           if constexpr (kSeam) {
-            if (q == nk - 1) y = y - aw * tw;
+            if (q == nk - 1) y = sub_prod(y, aw, tw);
           }
           // the k neighbour: the lane's previous step (the end row's value
           // before its first plane)
-          y = y - ak[u] * t_k;
-          y = y - aj[u] * (jj == 0 ? fd[u] : from_j);
-          y = y - ai[u] * (ii == 0 ? fl[u] : from_i);
+          y = sub_prod(y, ak[u], t_k);
+          y = sub_prod(y, aj[u], (jj == 0 ? fd[u] : from_j));
+          y = sub_prod(y, ai[u], (ii == 0 ? fl[u] : from_i));
           sh.v[rw[u]][lane] = y;
           const float tt = dd[u] * y;
           if constexpr (kSeam) {
@@ -684,13 +685,13 @@ __global__ void __launch_bounds__(64 * W)
           float from_i, from_j;
           lanes_from_upper(t_k, from_i, from_j);
           float sv = yy[u];
-          sv = sv - bi[u] * (ii == TI - 1 ? fr[u] : from_i);
-          sv = sv - bj[u] * (jj == TJ - 1 ? fu[u] : from_j);
+          sv = sub_prod(sv, bi[u], (ii == TI - 1 ? fr[u] : from_i));
+          sv = sub_prod(sv, bj[u], (jj == TJ - 1 ? fu[u] : from_j));
           // the k neighbour: the lane's previous step (the end row's value
           // before its first plane)
-          sv = sv - bk[u] * t_k;
+          sv = sub_prod(sv, bk[u], t_k);
           if constexpr (kSeam) {
-            if (q == 0) sv = sv - bw * tw;
+            if (q == 0) sv = sub_prod(sv, bw, tw);
           }
           const float zz = sv * dd[u];
           sh.v[rw[u]][lane] = zz;
@@ -727,6 +728,449 @@ __global__ void __launch_bounds__(64 * W)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register tiles (SweepPlan::stage_one == 2): the single-wave tile sweeps
+// with the tile held in VGPRs instead of LDS.  The sweeps' own arrays (dinv
+// and the FP32 y, tile-major) are stored skewed: lane l = ii + 8 jj keeps
+// plane q of its column in row (q + ii + jj) mod TK of the tile, so at march
+// step s every lane needs row s mod TK (forward) or (SKEW - 1 - s) mod TK
+// (backward), a compile-time register.  r (forward input) and z (backward
+// output) keep the solver's unskewed tile-major placement; each lane
+// rotates its own column by ii + jj planes in registers (four conditional
+// stages).  LDS holds only the per-plane factors and the incoming faces
+// (under 1.5 KB per tile), so the number of resident tiles per CU is set by
+// the VGPRs.  Point arithmetic and operation order are those of the other
+// schedules.
+//
+// Steps where a lane is outside its tile's planes leave its registers and
+// its k value unchanged (a select on s against ii + jj); the last tile of a
+// phi range is marched as a full TK-plane tile whose planes past the grid
+// are the zero padding of the tile-major arrays (r, y, dinv and the
+// per-plane factors all zero there, so they stay zero and feed zero).
+//
+// Faces: face row c of a tile holds, for each of its 8 entries, the value
+// the consuming edge lane needs at its march step c (mod TK), so a lane
+// reads row s mod TK at step s.  A producer's lane computes that value at
+// its own step c + 7, i.e. from its register (c + 7) mod TK (forward) or
+// (6 - c) mod TK (backward).  The k faces are the lane's k value after the
+// march (t of plane TK-1 forward, z of plane 0 backward) in face_k, and the
+// phi-seam values (t of plane 0 forward, z of plane c-1 backward) in
+// face_kt.
+template <int TK>
+__device__ __forceinline__ void rotate_col(float (&x)[TK], int m, int dir) {
+  // x[rho] <- x[(rho - dir*m) mod TK], m in [0, 15], in four conditional
+  // stages.  Both operands of each select pass through an empty asm, so that
+  // it stays a select of two values: a select of two array loads may be
+  // turned into one load from a selected address, which would keep x in
+  // memory (scratch) instead of registers.
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int sh = 1 << b;
+    const bool on = (m >> b) & 1;
+    float y[TK];
+#pragma unroll
+    for (int rr = 0; rr < TK; rr++) {
+      const int src = dir > 0 ? (rr - sh + TK) % TK : (rr + sh) % TK;
+      float moved = x[src], kept = x[rr];
+      asm("" : "+v"(moved), "+v"(kept));
+      y[rr] = on ? moved : kept;
+    }
+#pragma unroll
+    for (int rr = 0; rr < TK; rr++) x[rr] = y[rr];
+  }
+}
+
+// Per-plane factors of the register tiles: row u holds plane u - SKEW,
+// zero outside the tile's planes (TK + 2 SKEW <= 64 rows).
+template <int TK>
+struct TileRegLds {
+  float2 pf[TK + 2 * SKEW];
+  float fi[TK][TJ];  // incoming i face, consumer-step rows
+  float fj[TK][TI];  // incoming j face
+};
+
+// One register tile's operands, this lane's share, loaded in one batch.
+template <int TK>
+struct RegIn {
+  float v[TK];                  // forward: r of plane u (row u); backward: y, skewed
+  float d[TK];                  // dinv, skewed
+  float fi[TK / 8], fj[TK / 8];  // incoming faces: entries lane + 64 e
+  float2 pf;                    // per-plane factors of row `lane`
+  float fk;                     // k face of the neighbour tile in k
+  float fkt;                    // phi-seam value (phi-wrapped plans)
+  float ck, cj, ci;             // couplings of this lane's column
+};
+
+// Tile code on a diagonal: I + NI (J + NJ Kc).
+__device__ __forceinline__ int reg_tile_code(const DiagArgs& da, int b) {
+  if (da.tiles) return da.tiles[b];
+  int I, J, Kc;
+  diag_tile(da, b, I, J, Kc);
+  return I + da.NI * (J + da.NJ * Kc);
+}
+
+// Where a tile sits and what it couples to; one per tile, from its code.
+template <int TK>
+struct RegTile {
+  int I, J, Kc, k0, k1, nk;
+  size_t tid, col, tb;
+  __device__ __forceinline__ RegTile(const DiagArgs& da, const Grid3& g, int code, int lane) {
+    I = code % da.NI;
+    J = (code / da.NI) % da.NJ;
+    Kc = code / (da.NI * da.NJ);
+    k0 = Kc * TK;
+    k1 = min(g.c, k0 + TK);
+    nk = k1 - k0;
+    tid = (size_t)code;
+    col = (size_t)I + (size_t)da.NI * J;
+    tb = tid * (size_t)TK * 64 + lane;
+  }
+};
+
+// Stages a tile's faces and per-plane factors in the wave's LDS.  Only this
+// wave touches its LDS, and a wavefront's LDS operations are performed in
+// order, so wavefront-scope fences (compiler barriers, no instructions)
+// are all that orders these writes after the previous tile's reads and
+// before this tile's.
+template <int TK>
+__device__ __forceinline__ void reg_stage(TileRegLds<TK>& sh, const RegIn<TK>& in, int lane,
+                                          int nk, bool face_i, bool face_j) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+  for (int e = 0; e < TK / 8; e++) {
+    (&sh.fi[0][0])[lane + 64 * e] = face_i ? in.fi[e] : 0.f;
+    (&sh.fj[0][0])[lane + 64 * e] = face_j ? in.fj[e] : 0.f;
+  }
+  const int qf = lane - SKEW;
+  if (lane < TK + 2 * SKEW) sh.pf[lane] = (qf >= 0 && qf < nk) ? in.pf : make_float2(0.f, 0.f);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+template <typename T, int TK, bool kWrap>
+__device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, const Grid3& g,
+                                             const DiagArgs& da, const float* coup,
+                                             const T* __restrict__ r, int code, int lane) {
+  const RegTile<TK> t(da, g, code, lane);
+  const size_t fbi = tile_id(da, max(t.I - 1, 0), t.J, t.Kc) * TK * 8;
+  const size_t fbj = tile_id(da, t.I, max(t.J - 1, 0), t.Kc) * TK * 8;
+#pragma unroll
+  for (int e = 0; e < TK / 8; e++) {
+    in.fi[e] = da.face_i[fbi + lane + 64 * e];
+    in.fj[e] = da.face_j[fbj + lane + 64 * e];
+  }
+  const int qf = lane - SKEW;
+  const int kq = (qf >= 0 && qf < t.nk) ? t.k0 + qf : 0;
+  in.pf = make_float2(m.FDPH[kq], m.FDPL[kq]);
+  in.fk = da.face_k[tile_id(da, t.I, t.J, max(t.Kc - 1, 0)) * 64 + lane];
+  in.fkt = kWrap ? da.face_kt[t.col * 64 + lane] : 0.f;
+  in.ck = coup[(t.col * 3 + 0) * 64 + lane];
+  in.cj = coup[(t.col * 3 + 1) * 64 + lane];
+  in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+#pragma unroll
+  for (int u = 0; u < TK; u++) {
+    in.d[u] = da.dinv_t[t.tb + (size_t)u * 64];
+    in.v[u] = (float)r[t.tb + (size_t)u * 64];
+  }
+}
+
+template <typename T, int TK, bool kWrap>
+__device__ __forceinline__ void bwd_reg_load(RegIn<TK>& in, const Metric& m, const Grid3& g,
+                                             const DiagArgs& da, const float* coup, int code,
+                                             int lane) {
+  const RegTile<TK> t(da, g, code, lane);
+  const size_t fbi = tile_id(da, min(t.I + 1, da.NI - 1), t.J, t.Kc) * TK * 8;
+  const size_t fbj = tile_id(da, t.I, min(t.J + 1, da.NJ - 1), t.Kc) * TK * 8;
+#pragma unroll
+  for (int e = 0; e < TK / 8; e++) {
+    in.fi[e] = da.face_i[fbi + lane + 64 * e];
+    in.fj[e] = da.face_j[fbj + lane + 64 * e];
+  }
+  const int qf = lane - SKEW;
+  const int kq = (qf >= 0 && qf < t.nk) ? t.k0 + qf : 0;
+  in.pf = make_float2(m.FDPH[kq], m.FDPU[kq]);
+  in.fk = da.face_k[tile_id(da, t.I, t.J, min(t.Kc + 1, da.NK - 1)) * 64 + lane];
+  in.fkt = kWrap ? da.face_kt[t.col * 64 + lane] : 0.f;
+  in.ck = coup[(t.col * 3 + 0) * 64 + lane];
+  in.cj = coup[(t.col * 3 + 1) * 64 + lane];
+  in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+#pragma unroll
+  for (int u = 0; u < TK; u++) {
+    in.v[u] = da.yv_t[t.tb + (size_t)u * 64];
+    in.d[u] = da.dinv_t[t.tb + (size_t)u * 64];
+  }
+}
+
+template <int TK, bool kWrap>
+__device__ __forceinline__ void fwd_reg_tile(RegIn<TK>& in, TileRegLds<TK>& sh, const Metric& m,
+                                             const Grid3& g, const DiagArgs& da, int code,
+                                             int lane) {
+  const RegTile<TK> t(da, g, code, lane);
+  const int k0 = t.k0, k1 = t.k1, nk = t.nk;
+  const int ii = lane & (TI - 1), jj = lane / TI, dg = ii + jj;
+  const int i = t.I * TI + ii, j = t.J * TJ + jj;
+  const bool act = (i < g.a) && (j < g.b);
+  const bool lo_k0 = da.bl.lo_k(k0);
+  const bool wrap = kWrap && da.bl.wrap != 0;
+  const bool face_i = t.I > 0 && da.bl.lo_i(t.I * TI);
+  const bool face_j = t.J > 0 && da.bl.lo_j(t.J * TJ);
+  reg_stage(sh, in, lane, nk, face_i, face_j);
+  float (&rv)[TK] = in.v;
+  float (&dv)[TK] = in.d;
+  // r of plane q to register (q + ii + jj) mod TK
+  rotate_col<TK>(rv, dg, 1);
+  const float t_own = (act && lo_k0) ? in.fk : 0.f;
+  const float t_w = (act && wrap && k0 > 0 && k1 == g.c) ? in.fkt : 0.f;
+  const bool wrap_tile = wrap && k1 == g.c;  // seam coupling at plane nk-1
+  const bool cap0 = wrap && k0 == 0;         // records t of plane 0
+  const float aw = wrap_tile ? in.ck * m.FDPU[m.c - 1] : 0.f;
+  const float2* pfl = sh.pf + (SKEW - dg);  // pfl[s]: factors of plane s - dg
+  const float clk = in.ck, clj = in.cj, cli = in.ci;
+  float t_k = t_own, tw = t_w;
+  auto march = [&](auto seam) {
+    constexpr bool kSeam = decltype(seam)::value;
+#pragma unroll
+    for (int s = 0; s < TK + SKEW; s++) {
+      const int rho = s % TK;
+      const float2 f = pfl[s];
+      const float fl = sh.fi[rho][jj], fd = sh.fj[rho][ii];
+      float from_i, from_j;
+      lanes_from_lower(t_k, from_i, from_j);
+      const float ak = clk * f.y, aj = clj * f.x, ai = cli * f.x;
+      float y = rv[rho];
+      if constexpr (kSeam) {
+        if (wrap_tile && s - dg == nk - 1) y = sub_prod(y, aw, tw);
+      }
+      y = sub_prod(y, ak, t_k);
+      y = sub_prod(y, aj, (jj == 0 ? fd : from_j));
+      y = sub_prod(y, ai, (ii == 0 ? fl : from_i));
+      float tt = dv[rho] * y;
+      // every lane computes every step (the operands are read in batches,
+      // ahead of the dependency chain); only the results are selected
+      asm volatile("" : "+v"(y), "+v"(tt));
+      if constexpr (kSeam) {
+        if (cap0 && s == dg) tw = tt;
+      }
+      if (s < SKEW || s >= TK) {
+        const bool v = s < SKEW ? dg <= s : dg > s - TK;
+        rv[rho] = v ? y : rv[rho];
+        t_k = v ? tt : t_k;
+      } else {
+        rv[rho] = y;
+        t_k = tt;
+      }
+    }
+  };
+  // the seam steps are compiled only into the kernel for phi-wrapped plans
+  if constexpr (kWrap) {
+    if (wrap && (k0 == 0 || k1 == g.c))
+      march(std::true_type{});
+    else
+      march(std::false_type{});
+  } else {
+    march(std::false_type{});
+  }
+  // y outside the grid is stored as zero, so that the backward sweep's
+  // padding lanes (dinv zero) produce z = 0
+#pragma unroll
+  for (int u = 0; u < TK; u++) da.yv_t[t.tb + (size_t)u * 64] = act ? rv[u] : 0.f;
+  // outgoing faces t = dinv*y of the last column (ii = 7) and row (jj = 7)
+  const size_t fo = t.tid * TK * 8;
+  if (ii == TI - 1) {
+#pragma unroll
+    for (int c = 0; c < TK; c++)
+      da.face_i[fo + c * 8 + jj] = dv[(c + 7) % TK] * rv[(c + 7) % TK];
+  }
+  if (jj == TJ - 1) {
+#pragma unroll
+    for (int c = 0; c < TK; c++)
+      da.face_j[fo + c * 8 + ii] = dv[(c + 7) % TK] * rv[(c + 7) % TK];
+  }
+  da.face_k[t.tid * 64 + lane] = t_k;  // t of plane TK-1, for the tile above
+  if (cap0 && k1 < g.c) da.face_kt[t.col * 64 + lane] = tw;
+}
+
+template <typename T, int TK, bool kWrap>
+__device__ __forceinline__ void bwd_reg_tile(RegIn<TK>& in, TileRegLds<TK>& sh, const Metric& m,
+                                             const Grid3& g, const DiagArgs& da,
+                                             T* __restrict__ z, int code, int lane) {
+  const RegTile<TK> t(da, g, code, lane);
+  const int k0 = t.k0, k1 = t.k1, nk = t.nk;
+  const int ii = lane & (TI - 1), jj = lane / TI, dg = ii + jj, rdg = SKEW - dg;
+  const int i = t.I * TI + ii, j = t.J * TJ + jj;
+  const bool act = (i < g.a) && (j < g.b);
+  const bool hi_k1 = da.bl.hi_k(k1 - 1, g.c), wrap = kWrap && da.bl.wrap != 0;
+  const bool face_i = t.I + 1 < da.NI && da.bl.hi_i(t.I * TI + TI - 1, g.a);
+  const bool face_j = t.J + 1 < da.NJ && da.bl.hi_j(t.J * TJ + TJ - 1, g.b);
+  reg_stage(sh, in, lane, nk, face_i, face_j);
+  float (&yv)[TK] = in.v;
+  float (&dv)[TK] = in.d;
+  const float t_own = (act && hi_k1) ? in.fk : 0.f;
+  const float t_w = (act && wrap && k1 < g.c && k0 == 0) ? in.fkt : 0.f;
+  const bool wrap_tile = wrap && k0 == 0;    // seam coupling at plane 0
+  const bool cap_last = wrap && k1 == g.c;   // records z of plane nk-1
+  const float bw = wrap_tile ? in.ck * m.FDPL[0] : 0.f;
+  // step s handles plane q = TK-1 - s + rdg, factors in row q + SKEW
+  const float2* pfb = sh.pf + rdg;
+  const float cuk = in.ck, cuj = in.cj, cui = in.ci;
+  float t_k = t_own, tw = t_w;
+  auto march = [&](auto seam) {
+    constexpr bool kSeam = decltype(seam)::value;
+#pragma unroll
+    for (int s = 0; s < TK + SKEW; s++) {
+      const int rho = ((SKEW - 1 - s) % TK + TK) % TK;
+      const int crow = s % TK;
+      const float2 f = pfb[TK - 1 + SKEW - s];
+      const float fr = sh.fi[crow][jj], fu = sh.fj[crow][ii];
+      float from_i, from_j;
+      lanes_from_upper(t_k, from_i, from_j);
+      const float bk = cuk * f.y, bj = cuj * f.x, bi = cui * f.x;
+      float sv = yv[rho];
+      sv = sub_prod(sv, bi, (ii == TI - 1 ? fr : from_i));
+      sv = sub_prod(sv, bj, (jj == TJ - 1 ? fu : from_j));
+      sv = sub_prod(sv, bk, t_k);
+      const int q = TK - 1 - s + rdg;
+      if constexpr (kSeam) {
+        if (wrap_tile && q == 0) sv = sub_prod(sv, bw, tw);
+      }
+      float zz = sv * dv[rho];
+      asm volatile("" : "+v"(zz));  // computed on every step, as in the forward march
+      if constexpr (kSeam) {
+        if (cap_last && q == nk - 1) tw = zz;
+      }
+      if (s < SKEW || s >= TK) {
+        const bool v = s < SKEW ? rdg <= s : rdg > s - TK;
+        yv[rho] = v ? zz : yv[rho];
+        t_k = v ? zz : t_k;
+      } else {
+        yv[rho] = zz;
+        t_k = zz;
+      }
+    }
+  };
+  if constexpr (kWrap) {
+    if (wrap && (k0 == 0 || k1 == g.c))
+      march(std::true_type{});
+    else
+      march(std::false_type{});
+  } else {
+    march(std::false_type{});
+  }
+  // outgoing faces: z of the first column (ii = 0) and row (jj = 0)
+  const size_t fo = t.tid * TK * 8;
+  if (ii == 0) {
+#pragma unroll
+    for (int c = 0; c < TK; c++) da.face_i[fo + c * 8 + jj] = yv[((6 - c) % TK + TK) % TK];
+  }
+  if (jj == 0) {
+#pragma unroll
+    for (int c = 0; c < TK; c++) da.face_j[fo + c * 8 + ii] = yv[((6 - c) % TK + TK) % TK];
+  }
+  da.face_k[t.tid * 64 + lane] = t_k;  // z of plane 0, for the tile below
+  if (cap_last && k0 > 0) da.face_kt[t.col * 64 + lane] = tw;
+  // z of plane q from register (q + ii + jj) mod TK back to row q
+  rotate_col<TK>(yv, dg, -1);
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < TK; u++) z[t.tb + (size_t)u * 64] = (T)yv[u];
+  }
+}
+
+// Waves per SIMD the register tiles are built for: 7 forward and 8
+// backward (at most 72 and 64 VGPRs) for the FP32 16-plane tiles without
+// the phi seam, which then fit without spilling; the others take what they
+// need.
+template <typename T, int TK, bool kWrap, bool kFwd>
+constexpr int kRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? (kFwd ? 7 : 8) : 1;
+// Waves per workgroup: each marches a tile of its own (no data shared
+// between the waves), two per workgroup so that the resident tiles per CU
+// are not capped by the number of workgroups a CU holds.  (Looping a wave
+// over several tiles with the next tile's loads in flight measured slower:
+// the second operand set halves the resident waves.)
+constexpr int kRegTilesPerGroup = 2;
+
+template <typename T, int TK, bool kWrap>
+__global__ void __launch_bounds__(64 * kRegTilesPerGroup)
+    __attribute__((amdgpu_waves_per_eu(kRegWaves<T, TK, kWrap, true>, 8)))
+    k_tile_fwd_reg(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ coup,
+                   const T* __restrict__ r, const CgState* guard) {
+  static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
+  __shared__ TileRegLds<TK> shw[kRegTilesPerGroup];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = (int)blockIdx.x * kRegTilesPerGroup + wave;
+  if (b >= da.count) return;  // no workgroup barrier follows
+  const int code = reg_tile_code(da, b);
+  RegIn<TK> in;
+  fwd_reg_load<T, TK, kWrap>(in, m, g, da, coup, r, code, lane);
+  // the done flag is read after the tile's loads, before any store
+  if (guard && st_done(guard)) return;
+  fwd_reg_tile<TK, kWrap>(in, shw[wave], m, g, da, code, lane);
+}
+
+template <typename T, int TK, bool kWrap>
+__global__ void __launch_bounds__(64 * kRegTilesPerGroup)
+    __attribute__((amdgpu_waves_per_eu(kRegWaves<T, TK, kWrap, false>, 8)))
+    k_tile_bwd_reg(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ coup,
+                   T* __restrict__ z, const CgState* guard) {
+  static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
+  __shared__ TileRegLds<TK> shw[kRegTilesPerGroup];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = (int)blockIdx.x * kRegTilesPerGroup + wave;
+  if (b >= da.count) return;  // no workgroup barrier follows
+  const int code = reg_tile_code(da, b);
+  RegIn<TK> in;
+  bwd_reg_load<T, TK, kWrap>(in, m, g, da, coup, code, lane);
+  if (guard && st_done(guard)) return;
+  bwd_reg_tile<T, TK, kWrap>(in, shw[wave], m, g, da, z, code, lane);
+}
+
+// Couplings of every column of tiles for the register tiles: lower (k, j,
+// i) at coup_lo[(col * 3 + c) * 64 + lane], upper likewise, col = I + TNI J;
+// columns outside the grid take those of column (0, 0), as in the marches.
+__global__ void k_reg_coup(Metric m, Grid3 g, int TNI, int TNJ, float* __restrict__ lo,
+                           float* __restrict__ up) {
+  const int col = blockIdx.x, lane = threadIdx.x;
+  const int I = col % TNI, J = col / TNI;
+  const int i = I * TI + (lane & 7), j = J * TJ + (lane >> 3);
+  const bool act = i < g.a && j < g.b;
+  const SweepCoup cl = sweep_coup_lower(m, act ? i : 0, act ? j : 0);
+  const SweepCoup cu = sweep_coup_upper(m, act ? i : 0, act ? j : 0);
+  float* l = lo + (size_t)col * 192 + lane;
+  float* u = up + (size_t)col * 192 + lane;
+  l[0] = cl.k;
+  l[64] = cl.j;
+  l[128] = cl.i;
+  u[0] = cu.k;
+  u[64] = cu.j;
+  u[128] = cu.i;
+}
+
+template <typename T, int TK>
+void tiles_launch_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, const T* r, T* z,
+                      const CgState* guard, hipStream_t s) {
+  const bool wrap = plan.bl.wrap != 0;
+  constexpr int kThreads = 64 * kRegTilesPerGroup;
+  for (int L = 0; L < plan.ndiag; L++) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    const DiagArgs da = diag_args(plan, L);
+    const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
+    if (wrap)
+      k_tile_fwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+    else
+      k_tile_fwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+  }
+  for (int L = plan.ndiag - 1; L >= 0; L--) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    const DiagArgs da = diag_args(plan, L);
+    const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
+    if (wrap)
+      k_tile_bwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
+    else
+      k_tile_bwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
+  }
+}
+
 template <typename T, int TK, int W>
 void tiles_launch_lds_w(const SweepPlan& plan, const Metric& m, const Grid3& g, const T* r, T* z,
                         const CgState* guard, hipStream_t s) {
@@ -746,10 +1190,13 @@ void tiles_launch_lds_tk(const SweepPlan& plan, const Metric& m, const Grid3& g,
                          hipStream_t s) {
   (void)dinv;
   (void)yv;  // the single-wave sweeps use the plan's tile-major copies
-  if (plan.stage_one)
+  if (plan.stage_one == 2) {
+    if constexpr (TK + 2 * SKEW <= 64) tiles_launch_reg<T, TK>(plan, m, g, r, z, guard, s);
+  } else if (plan.stage_one) {
     tiles_launch_lds_w<T, TK, 1>(plan, m, g, r, z, guard, s);
-  else
+  } else {
     tiles_launch_lds_w<T, TK, TK / 8>(plan, m, g, r, z, guard, s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -862,10 +1309,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
         if (ii == 0) fl = (si == 0) ? sh.fi[q][8 * sj + jj] : sh.xi[si > 0 ? si - 1 : 0][sj][q][jj];
         if (jj == 0) fd = (sj == 0) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj > 0 ? sj - 1 : 0][q][ii];
         float y = rr[u];
-        if (wrap && k == g.c - 1) y = y - aw * t_w;
-        if (q > 0 || lo_k0) y = y - ak[u] * t_own;
-        if (lo_j) y = y - aj[u] * fd;
-        if (lo_i) y = y - ai[u] * fl;
+        if (wrap && k == g.c - 1) y = sub_prod(y, aw, t_w);
+        if (q > 0 || lo_k0) y = sub_prod(y, ak[u], t_own);
+        if (lo_j) y = sub_prod(y, aj[u], fd);
+        if (lo_i) y = sub_prod(y, ai[u], fl);
         sh.v[q][tid] = y;
         const float tt = dd[u] * y;
         if (ii == 7 && si < SI - 1) sh.xi[si][sj][q][jj] = tt;
@@ -971,10 +1418,10 @@ __global__ void __launch_bounds__(64 * SI * SJ)
         if (ii == 7) fr = (si == SI - 1) ? sh.fi[q][8 * sj + jj] : sh.xi[si < SI - 1 ? si : 0][sj][q][jj];
         if (jj == 7) fu = (sj == SJ - 1) ? sh.fj[q][8 * si + ii] : sh.xj[si][sj < SJ - 1 ? sj : 0][q][ii];
         float sv = yy[u];
-        if (hi_i) sv = sv - bi[u] * fr;
-        if (hi_j) sv = sv - bj[u] * fu;
-        if (q < nk - 1 || hi_k1) sv = sv - bk[u] * t_own;
-        if (wrap && k == 0) sv = sv - bw * t_w;
+        if (hi_i) sv = sub_prod(sv, bi[u], fr);
+        if (hi_j) sv = sub_prod(sv, bj[u], fu);
+        if (q < nk - 1 || hi_k1) sv = sub_prod(sv, bk[u], t_own);
+        if (wrap && k == 0) sv = sub_prod(sv, bw, t_w);
         const float zz = sv * dd[u];
         sh.v[q][tid] = zz;
         if (ii == 0 && si > 0) sh.xi[si - 1][sj][q][jj] = zz;
@@ -1055,13 +1502,16 @@ void tiles_launch(const SweepPlan& plan, const CS& cs, const Grid3& g, const flo
 // dinv (natural order) -> tile-major copy: element t of tile (I, J, Kc),
 // plane q, lane ii + 8 jj holds point (8I + ii, 8J + jj, TK Kc + q); zero
 // where that point is outside the grid.
-__global__ void k_tile_gather(Grid3 g, int TNI, int TNJ, int TK, size_t n,
+// dst[tile][row][lane] = src at plane q of the tile, q = row, or with skew
+// q = (row - ii - jj) mod TK (the register tiles' placement).
+__global__ void k_tile_gather(Grid3 g, int TNI, int TNJ, int TK, int skew, size_t n,
                               const float* __restrict__ src, float* __restrict__ dst) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   const int lane = (int)(t & 63);
-  const size_t pq = t >> 6;  // tile * TK + q
-  const int q = (int)(pq % (size_t)TK);
+  const size_t pq = t >> 6;  // tile * TK + row
+  const int row = (int)(pq % (size_t)TK);
+  const int q = skew ? (row - (lane & 7) - (lane >> 3) + 2 * TK) % TK : row;
   const size_t tile = pq / (size_t)TK;
   const int I = (int)(tile % (size_t)TNI);
   const int J = (int)((tile / (size_t)TNI) % (size_t)TNJ);
@@ -1073,12 +1523,17 @@ __global__ void k_tile_gather(Grid3 g, int TNI, int TNJ, int TK, size_t n,
 }
 
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, int tile_waves,
-                     const IluBlocks* blk, const float* dinv, int stage_one) {
+                     const IluBlocks* blk, const float* dinv, int stage_one, const Metric* m) {
   free_sweep_plan(plan);
   if (g.c < 2) return 1;
   // tile diagonals
-  plan.stage_one = stage_one ? 1 : 0;
+  plan.stage_one = (stage_one == 1 || stage_one == 2) ? stage_one : 0;
   plan.TK = (tile_k == 16 || tile_k == 64) ? tile_k : kTileK;
+  // register tiles: 8x8 columns, at most 32 planes (TK + 2 SKEW <= 64)
+  if (plan.stage_one == 2 && (tile_waves != 1 || plan.TK + 2 * SKEW > 64 || !m)) {
+    free_sweep_plan(plan);
+    return 3;
+  }
   plan.SI = plan.SJ = 1;
   if (tile_waves == 4) plan.SI = plan.SJ = 2;
   if (tile_waves == 8) {
@@ -1143,9 +1598,19 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     }
     if (dinv) {
       k_tile_gather<<<(unsigned)((plan.tile_elems + 255) / 256), 256>>>(
-          g, plan.TNI, plan.TNJ, plan.TK, plan.tile_elems, dinv, plan.dinv_t);
+          g, plan.TNI, plan.TNJ, plan.TK, plan.stage_one == 2, plan.tile_elems, dinv,
+          plan.dinv_t);
     } else {
       hipMemset(plan.dinv_t, 0, plan.tile_elems * sizeof(float));
+    }
+    if (plan.stage_one == 2) {
+      const size_t ncol = (size_t)plan.TNI * plan.TNJ;
+      if (hipMalloc((void**)&plan.coup_lo, ncol * 192 * sizeof(float)) != hipSuccess ||
+          hipMalloc((void**)&plan.coup_up, ncol * 192 * sizeof(float)) != hipSuccess) {
+        free_sweep_plan(plan);
+        return 1;
+      }
+      k_reg_coup<<<(unsigned)ncol, 64>>>(*m, g, plan.TNI, plan.TNJ, plan.coup_lo, plan.coup_up);
     }
     if (hipDeviceSynchronize() != hipSuccess) {
       free_sweep_plan(plan);
@@ -1164,6 +1629,8 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.yv_t) hipFree(plan.yv_t);
   if (plan.face_k) hipFree(plan.face_k);
   if (plan.face_kt) hipFree(plan.face_kt);
+  if (plan.coup_lo) hipFree(plan.coup_lo);
+  if (plan.coup_up) hipFree(plan.coup_up);
   plan = SweepPlan();
 }
 

@@ -40,8 +40,15 @@ struct SweepPlan {
   float* face_k = nullptr;   // per tile: backward z of its first plane
   float* face_kt = nullptr;  // per column of tiles: backward z of plane c-1
   // single-wave tiles: 1 when the marching wave stages its tile alone (one
-  // wave and about 20 KB of LDS per tile), 0 when TK/8 waves stage it
+  // wave and about 20 KB of LDS per tile), 0 when TK/8 waves stage it, 2
+  // when the marching wave holds its tile in registers (one wave, about
+  // 2.4 KB of LDS; dinv_t and yv_t then hold plane q of lane ii + 8 jj in
+  // row (q + ii + jj) mod TK)
   int stage_one = 0;
+  // register tiles: per-column couplings (lower, upper), 3 x 64 floats per
+  // column of tiles
+  float* coup_lo = nullptr;
+  float* coup_up = nullptr;
 };
 
 // Returns 0 when the tile sweeps can run on this grid.
@@ -51,10 +58,12 @@ struct SweepPlan {
 // multiples of the tile extents (returns nonzero otherwise).
 // dinv: the D-ILU's inverse pivots (natural order), copied tile-major for
 // the single-wave tile sweeps (build the plan again after refactoring).
-// stage_one: see SweepPlan::stage_one.
+// stage_one: see SweepPlan::stage_one; 2 needs tile_waves 1, tile_k 16 or
+// 32 and the operator's metric m (returns 3 otherwise).
 int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k = 32,
                      int tile_waves = 1, const IluBlocks* bl = nullptr,
-                     const float* dinv = nullptr, int stage_one = 0);
+                     const float* dinv = nullptr, int stage_one = 0,
+                     const Metric* m = nullptr);
 void free_sweep_plan(SweepPlan& plan);
 // Tile-diagonal sweeps: z = (LU)^-1 r with 2*ndiag launches and no waiting
 // between workgroups (the schedule used inside the CG iteration).

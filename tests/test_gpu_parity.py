@@ -165,7 +165,7 @@ def test_ilu_tile_sweeps_match_hyperplanes(solver_mod, dims):
             # single-wave tiles staged by the marching wave alone
             for waves, tk, stage in ((1, 16, 0), (1, 64, 0), (4, 16, 0), (4, 32, 0),
                                      (4, 64, 0), (8, 16, 0), (8, 32, 0), (1, 16, 1),
-                                     (1, 32, 1), (1, 64, 1)):
+                                     (1, 32, 1), (1, 64, 1), (1, 16, 2), (1, 32, 2)):
                 s.set_option(_lib.OPT_TILE_WAVES, waves)
                 s.set_option(_lib.OPT_TILE_K, tk)
                 s.set_option(_lib.OPT_TILE_STAGE, stage)
@@ -401,6 +401,15 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
         assert _rel(x, x64) < 1e-8
         s.fill_phi()
         assert np.allclose(s.magnetic_energy(), w64, rtol=1e-7, atol=0)
+        # the same FP32 solve with the tiles held in registers: the same
+        # arithmetic per point, so the same iterates bit for bit
+        s.set_option(_lib.OPT_TILE_STAGE, 2)
+        x2 = np.zeros(s.N)
+        assert s.solve(x2, None, 1e-9, ncgmax=5000)[0] == 0
+        st2 = s.stats()
+        say(f"  register tiles: ncg={st2['ncg']} replacements={st2['n_outer']}")
+        assert st2["ncg"] == st["ncg"] and st2["n_outer"] == st["n_outer"]
+        assert np.array_equal(x2, x)
 
 
 def test_failure_paths(oracle_mod, solver_mod):
@@ -599,7 +608,7 @@ def test_block_jacobi_ilu(oracle_mod, solver_mod, nblk):
         z_lv = s.prec_inv(x)
         s.set_option(_lib.OPT_SWEEP, 0)
         for waves, tk, stage in ((1, 32, 0), (1, 64, 0), (4, 64, 0), (8, 32, 0), (1, 16, 1),
-                                 (1, 32, 1)):
+                                 (1, 32, 1), (1, 16, 2), (1, 32, 2)):
             s.set_option(_lib.OPT_TILE_WAVES, waves)
             s.set_option(_lib.OPT_TILE_K, tk)
             s.set_option(_lib.OPT_TILE_STAGE, stage)
