@@ -143,11 +143,12 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
 
 
 # Kernels of the value path's iteration: FP32 recurrence with the
-# block-Jacobi ILU0 on single-wave tiles.  (name, time_kernel id,
-# algorithmic bytes per unknown, padded-array bytes per point)
+# block-Jacobi ILU0 on register tiles.  (name, time_kernel id, algorithmic
+# bytes per unknown, padded-array bytes per point); the FP32 operator's
+# kernel is the library's automatic choice for the grid (STENCIL_FP32).
 VALUE_KERNELS = [
-    ("ilu_apply_fp32_blockjacobi (k_tile_fwd_lds+k_tile_bwd_lds<float>)", 7, 24, 0),
-    ("stencil_fp32 (k_stencil_march<float>)", 1, 4, 4),
+    ("ilu_apply_fp32_blockjacobi (k_tile_fwd_reg+k_tile_bwd_reg<float>)", 7, 24, 0),
+    ("STENCIL_FP32", 1, 4, 4),
     # p, z (tile-major, gathered), FP64 solution: 4+4+8 read, 4+8 written
     ("p_update_fp32_fp64x (k_p_update<float>)", 8, 28, 0),
     # r, ap read, r written (tile-major)
@@ -169,12 +170,24 @@ def set_fast_ilu(s, args):
     return blocks
 
 
-def measure_value_kernels(s, N: int, padded: int, reps: int = 5):
+def stencil_fp32_kernel(nr: int) -> str:
+    """The FP32 operator kernel the library chooses by default (P3D_OPT_STENCIL
+    = -1): the row-blocked march when its 62-lane waves cover the nr-2 radial
+    points to at least 90%, else the column march."""
+    a = nr - 2
+    nwi = -(-a // 62)
+    rows = 10 * a >= 9 * nwi * 62
+    return "stencil_fp32 (k_stencil_rows<float>)" if rows else "stencil_fp32 (k_stencil_march<float>)"
+
+
+def measure_value_kernels(s, N: int, padded: int, nr: int, reps: int = 5):
     """The value path's kernels (set_fast_ilu applied), timed as
     measure_kernels; the ILU apply is the dominant one, so it is the
     roofline kernel.  Also the algorithmic bytes of one iteration."""
     table = {}
     for name, kid, per_n, per_pad in VALUE_KERNELS:
+        if name == "STENCIL_FP32":
+            name = stencil_fp32_kernel(nr)
         ms = s.time_kernel(kid, reps)
         nbytes = per_n * N + per_pad * padded
         table[name] = {"ms": round(ms, 5), "bytes": nbytes,
@@ -230,9 +243,9 @@ def main():
                          "1,1,1 = single-rank ILU0")
     ap.add_argument("--tile-k", type=int, default=16,
                     help="planes per ILU tile of the block-Jacobi fast paths (16, 32, 64)")
-    ap.add_argument("--tile-stage", type=int, default=1,
-                    help="ILU tiles of the fast paths staged by the marching wave alone (1) "
-                         "or by planes/8 waves (0)")
+    ap.add_argument("--tile-stage", type=int, default=2,
+                    help="ILU tiles of the fast paths held in registers (2), staged in LDS "
+                         "by the marching wave alone (1) or by planes/8 waves (0)")
     ap.add_argument("--levels", action="store_true",
                     help="also time the FP64 solve with hyperplane-level trisolves")
     ap.add_argument("--guess-check", type=int, default=2,
@@ -271,7 +284,7 @@ def main():
     if args.kernels_only:
         kern = measure_kernels(s, N, nr * nt * np_, reps=5, graph=False)
         set_fast_ilu(s, args)
-        vkern = measure_value_kernels(s, N, nr * nt * np_)
+        vkern = measure_value_kernels(s, N, nr * nt * np_, nr)
         print(json.dumps({"workload": w.name, "unknowns": N, "kernels": kern["table"],
                           "value_path_kernels": vkern["table"]}))
         s.close()
@@ -338,7 +351,7 @@ def main():
         s.set_option(_lib.OPT_SWEEP, 0)
     kern = measure_kernels(s, N, nr * nt * np_)
     blocks = set_fast_ilu(s, args)
-    vkern = measure_value_kernels(s, N, nr * nt * np_)
+    vkern = measure_value_kernels(s, N, nr * nt * np_, nr)
     tbj, stbj, _ = timed(f"fp64 block-Jacobi{blocks} ILU0 PCG", 0, steps=args.ref_runs)
 
     def errs(x):

@@ -128,10 +128,12 @@ enum {
                                 2 the marching wave holds its tile in
                                 registers (16 or 32 planes, one wave per
                                 tile).  Same results bit for bit. */
-  P3D_OPT_STENCIL = 15,     /* operator kernel: 0 (default) one thread per
-                                (i,j) column marching phi; 1 each thread
-                                holds 8 theta rows of one column, the
-                                neighbours in i from the adjacent lanes.
+  P3D_OPT_STENCIL = 15,     /* operator kernel: 0 one thread per (i,j)
+                                column marching phi; 1 each thread holds 8
+                                theta rows of one column, the neighbours in
+                                i from the adjacent lanes; -1 (default) 1 for
+                                the FP32 operator when its 62-lane waves
+                                cover the r extent to 90% or more, else 0.
                                 Same operator values bit for bit. */
   P3D_OPT_APPLY_FP32 = 16   /* p3d_apply_A: 1 applies the FP32 operator of
                                 the mixed-precision recurrence (x rounded to

@@ -397,6 +397,17 @@ __host__ __device__ inline void stencil_rows_shape(const Grid3& g, int& nwi, int
   nwg = ((g.b + kRows - 1) / kRows + 3) / 4;
 }
 
+// The kernel the automatic choice takes: the row-blocked march for the
+// FP32 operator when its 62-lane waves cover the r extent to at least 90%
+// (measured faster there: 0.61 vs 0.71 ms at 299 x 599 x 1199; slower at
+// 149 x 299 x 599, where a third of the third wave idles), else the column
+// march (faster for the FP64 operator at both sizes).
+inline bool stencil_rows_chosen(const Grid3& g, size_t elem_bytes) {
+  if (elem_bytes != 4) return false;
+  const int nwi = (g.a + kRowLanes - 1) / kRowLanes;
+  return 10 * g.a >= 9 * nwi * kRowLanes;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_stencil_rows(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
@@ -545,7 +556,7 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     int slot, Epi epi, double* out, bool guard, hipStream_t s,
                     const TileMap& tm) {
   const int want = (epi != EPI_NONE);
-  if (L.stencil == 1) {
+  if (L.stencil == 1 || (L.stencil < 0 && stencil_rows_chosen(g, sizeof(T)))) {
     const int nseg = stencil_rows_segments(g);
     const int G = stencil_rows_blocks(g);
     k_stencil_rows<T><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, b, mode, pole_part, st,

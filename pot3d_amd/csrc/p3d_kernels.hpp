@@ -89,7 +89,9 @@ __device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, d
 struct Launch {
   int G;        // blocks of a streaming kernel (multiple of 8)
   int CH;       // points per block (multiple of 256)
-  int stencil = 0;  // operator kernel: 0 column march, 1 row-blocked march
+  // operator kernel: 0 column march, 1 row-blocked march, -1 chosen per
+  // precision and grid (stencil_rows_chosen)
+  int stencil = -1;
 };
 
 constexpr int kBlock = 256;

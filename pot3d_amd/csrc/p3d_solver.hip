@@ -963,7 +963,8 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       break;
     case P3D_OPT_STENCIL: {
       const int v = (int)value;
-      if (v != 0 && v != 1) return fail(P3D_EINVAL, "P3D_OPT_STENCIL must be 0 or 1");
+      if (v != -1 && v != 0 && v != 1)
+        return fail(P3D_EINVAL, "P3D_OPT_STENCIL must be -1, 0 or 1");
       if (v != s->L.stencil) {
         // captured iterations hold the other kernel
         HIPCHK(hipStreamSynchronize(s->stream));

@@ -147,7 +147,7 @@ def _fp64_then_mixed(w, blocks, label, oracle_check=None):
         B64 = s.getb(phi64)
         w64 = s.magnetic_energy(phi64)
         _set(s, ilu_blocks_r=blocks[0], ilu_blocks_t=blocks[1], ilu_blocks_p=blocks[2],
-             tile_k=16, tile_stage=1, mixed=2)
+             tile_k=16, tile_stage=2, mixed=2)
         x = np.zeros(w.N)
         ierr, ncg, _ = s.solve(x, None, 1e-9, ncgmax=20000)
         st = s.stats()
@@ -212,7 +212,7 @@ def test_cfg4_learned_warm_start_real_grids():
     with solver.Pot3dSolver(w.mesh, w.pm_r1, 2) as s:
         s.set_br0(w.br0)
         _set(s, stop=1, mixed=2, ilu_blocks_r=8, ilu_blocks_t=16, ilu_blocks_p=16, tile_k=16,
-             tile_stage=1)
+             tile_stage=2)
         x0 = np.zeros(w.N)
         assert s.solve(x0, None, 1e-9, ncgmax=20000)[0] == 0
         n0 = s.stats()["ncg"]
