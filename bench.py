@@ -376,7 +376,7 @@ def main():
     result_ws = None
     if use_net:
         from pot3d_amd import warmstart as W
-        net, meta = W.WarmStartNet.load(wpath, wc.mesh)
+        net, meta = W.WarmStartNet.load(wpath, wc.mesh, wc.pm_r1)
         sc = solver.Pot3dSolver(wc.mesh, wc.pm_r1, 2)
         # the fine problem's Br map (seed 0) on the coarse mesh
         brc = wc.br0

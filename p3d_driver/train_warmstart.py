@@ -12,6 +12,10 @@ its weights (python -m pot3d_amd.train_warmstart --domain pfss|open).
    stepping down to lr/4 and lr/16 for the last 40% and 15% of the steps.
 4. The last `val` pairs are held out: their guess error and the PCG
    iterations from zero and from the guess are stored with the weights.
+
+With --monopole 1 (the default for 'open', whose |Br| maps have a large
+mean) the network learns the maps' non-monopole part only; the monopole's
+response comes from one PCG solve for br0 = 1 (pot3d_amd.warmstart).
 """
 from __future__ import annotations
 
@@ -64,6 +68,9 @@ def main(argv=None):
     ap.add_argument("--augment", type=int, default=1)
     ap.add_argument("--init-seed", type=int, default=0)
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--monopole", type=int, default=-1,
+                    help="split br0's area-weighted mean off (1), or not (0); default 1 for "
+                         "open, 0 for pfss")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -104,7 +111,11 @@ def main(argv=None):
     va_b, va_x = brs[ntr:], xs[ntr:]
 
     net = W.WarmStartNet(w0.mesh, seed=args.init_seed)
-    net.set_scales(*W.fit_scales(tr_b, tr_x))
+    mono = args.monopole if args.monopole >= 0 else int(args.domain == "open")
+    if mono:
+        net.set_monopole(W.monopole_response(w0.mesh, w0.pm_r1), W.area_weights(w0.mesh))
+    net.set_scales(*W.fit_scales((net.split(b)[0] for b in tr_b),
+                                 (net.split(b, x)[1] for b, x in zip(tr_b, tr_x))))
     for b, x in zip(tr_b, tr_x):
         net.add_sample(b, x)
     aug = (uniform_phi(w0.mesh), symmetric_theta(w0.mesh), True) if args.augment else (0, 0, 0)

@@ -13,6 +13,13 @@ Networks are trained on a coarse grid and their prediction is carried to a
 finer grid by trilinear prolongation of the padded coarse potential (with
 its boundary values and seam, p3d_fill_phi) on the GPU.
 
+The area-weighted mean m of br0 (the monopole, large for the 'open'
+problem's |Br| maps) is a global quantity that the network's receptive field
+does not span.  With set_monopole the network sees br0 - m and learns
+x - m x_m, where x_m is the PCG solution for br0 = 1 on the same grid: the
+problem is linear in br0, so the prediction m x_m + net(br0 - m) carries
+the monopole exactly.
+
 Every computation runs through the C ABI; there is no CPU path.
 """
 from __future__ import annotations
@@ -55,6 +62,7 @@ class WarmStartNet:
         self.nsamples = 0
         self.bscale = 1.0
         self.phiscale = 1.0
+        self._mono = None  # (x_m, area weights of br0's interior) with the monopole split
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -81,7 +89,34 @@ class WarmStartNet:
         assert p.size == self.nparams
         check(self._L.p3d_nn_set_params(self._h, p.ctypes.data_as(_fp)))
 
+    def set_monopole(self, x_m: np.ndarray, area: np.ndarray):
+        """Split br0's area-weighted mean off (see the module docstring):
+        x_m the packed solution for br0 = 1, area the (nt-2, np-2) weights of
+        br0's interior points (area_weights)."""
+        x_m = np.ascontiguousarray(x_m, dtype=np.float64)
+        area = np.asarray(area, dtype=np.float64)
+        assert x_m.size == self.N and area.shape == (self.nt - 2, self.np - 2)
+        self._mono = (x_m, area)
+
+    @property
+    def monopole(self) -> bool:
+        return self._mono is not None
+
+    def mean_br(self, br0: np.ndarray) -> float:
+        """Area-weighted mean of br0 over its interior points."""
+        area = self._mono[1]
+        return float(np.sum(area * np.asarray(br0)[1:-1, 1:-1]) / np.sum(area))
+
+    def split(self, br0: np.ndarray, x: np.ndarray | None = None):
+        """(br0 - m, x - m x_m) with the monopole split, else the inputs."""
+        if self._mono is None:
+            return br0, x
+        m = self.mean_br(br0)
+        return (np.asarray(br0) - m,
+                None if x is None else np.asarray(x) - m * self._mono[0])
+
     def add_sample(self, br0: np.ndarray, x: np.ndarray):
+        br0, x = self.split(br0, x)
         b = _fortran_br(br0, self.nt, self.np)
         xx = np.ascontiguousarray(x, dtype=np.float64)
         assert xx.size == self.N
@@ -89,6 +124,7 @@ class WarmStartNet:
         self.nsamples += 1
 
     def loss_grad(self, br0: np.ndarray, x: np.ndarray, grads: bool = True):
+        br0, x = self.split(br0, x)
         b = _fortran_br(br0, self.nt, self.np)
         xx = np.ascontiguousarray(x, dtype=np.float64)
         loss = C.c_double(0.0)
@@ -102,15 +138,20 @@ class WarmStartNet:
                                          int(sign_flip)))
 
     def save(self, path: str, **meta):
-        """Parameters, scales and the grid (plus ``meta``) in an .npz file."""
+        """Parameters, scales and the grid (plus ``meta``) in an .npz file;
+        with the monopole split, monopole = 1 (x_m is recomputed on load)."""
+        if self._mono is not None:
+            meta = dict(meta, monopole=1)
         np.savez_compressed(path, params=self.params(), bscale=self.bscale,
                             phiscale=self.phiscale, grid=np.array([self.nr, self.nt, self.np]),
                             **meta)
 
     @classmethod
-    def load(cls, path: str, mesh):
+    def load(cls, path: str, mesh, pm_r1: float | None = None):
         """A network for ``mesh`` with the parameters and scales of ``path``
-        (written by save); returns (net, the file's other entries)."""
+        (written by save); returns (net, the file's other entries).  A network
+        trained with the monopole split needs the problem's pm_r1 (the outer
+        boundary condition) to solve for x_m."""
         z = np.load(path, allow_pickle=False)
         grid = tuple(int(v) for v in z["grid"])
         if grid != (mesh.nr, mesh.nt, mesh.np):
@@ -119,6 +160,11 @@ class WarmStartNet:
         net = cls(mesh, seed=0)
         net.set_params(z["params"])
         net.set_scales(float(z["bscale"]), float(z["phiscale"]))
+        if "monopole" in z.files and int(z["monopole"]):
+            if pm_r1 is None:
+                net.close()
+                raise ValueError(f"{path} splits the monopole off: pass the problem's pm_r1")
+            net.set_monopole(monopole_response(mesh, pm_r1), area_weights(mesh))
         meta = {k: z[k] for k in z.files if k not in ("params", "bscale", "phiscale", "grid")}
         return net, meta
 
@@ -129,14 +175,40 @@ class WarmStartNet:
         return losses[:nsteps]
 
     def predict(self, br0: np.ndarray) -> np.ndarray:
-        b = _fortran_br(br0, self.nt, self.np)
+        bs, _ = self.split(br0)
+        b = _fortran_br(bs, self.nt, self.np)
         x = np.empty(self.N)
         check(self._L.p3d_nn_predict(self._h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), None))
+        if self._mono is not None:
+            x += self.mean_br(br0) * self._mono[0]
         return x
 
     def predict_dev(self, br0: np.ndarray, x_ptr: int):
+        if self._mono is not None:
+            raise ValueError("predict_dev has no monopole split: use predict")
         b = _fortran_br(br0, self.nt, self.np)
         check(self._L.p3d_nn_predict(self._h, b.ctypes.data_as(_dp), None, C.c_void_p(x_ptr)))
+
+
+def area_weights(mesh) -> np.ndarray:
+    """Cell areas sin(theta) dtheta dphi of br0's interior points (nt-2, np-2):
+    the weights of the driver's flux sums."""
+    return (np.asarray(mesh.sth)[1:-1, None] * np.asarray(mesh.dth)[1:-1, None]
+            * np.asarray(mesh.dph)[None, 1:-1])
+
+
+def monopole_response(mesh, pm_r1: float, epscg: float = 1e-12) -> np.ndarray:
+    """x_m: the packed FP64 PCG solution for br0 = 1 on ``mesh`` (ILU0,
+    ||b-Ax||2/||b||2 <= epscg)."""
+    from .solver import Pot3dSolver
+    with Pot3dSolver(mesh, pm_r1, 2) as s:
+        s.set_br0(np.ones((mesh.nt, mesh.np)))
+        s.set_option(_lib.OPT_STOP, 1)
+        x = np.zeros(s.N)
+        ierr, ncg, _ = s.solve(x, None, epscg, ncgmax=100000)
+        if ierr != 0:
+            raise RuntimeError(f"monopole response did not converge ({ncg} iterations)")
+    return x
 
 
 # -- prolongation ------------------------------------------------------------
@@ -242,8 +314,9 @@ def train_network(net: WarmStartNet, brs, xs, epochs: int = 4, lr: float = 1e-3,
     steps, the learning rate stepping down to lr/4 and lr/16 for the last
     40% and 15% of the steps; returns the per-step losses and the training
     time."""
-    if set_scales:
-        net.set_scales(*fit_scales(brs, xs))
+    if set_scales:  # on what the network sees (after the monopole split, if any)
+        net.set_scales(*fit_scales((net.split(b)[0] for b in brs),
+                                   (net.split(b, x)[1] for b, x in zip(brs, xs))))
     for b, x in zip(brs, xs):
         net.add_sample(b, x)
     total = epochs * len(brs)

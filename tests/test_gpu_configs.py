@@ -186,24 +186,27 @@ def test_cfg5_open_mixed_vs_fp64():
     _fp64_then_mixed(w, (8, 16, 16), "cfg5")
 
 
-def test_cfg4_learned_warm_start_real_grids():
-    """cfg4: the network trained on 51x91x181 PCG solutions (committed
-    weights, held-out seed 0) predicts on 51x91x181, is prolonged to
-    151x301x601 and starts the value path's solve (mixed precision,
-    block-Jacobi ILU0): the guess is kept, the answer is the zero guess's
-    (phi, B, energy) and the solve takes fewer iterations."""
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("cfg,family", [(4, "pfss"), (5, "open")])
+def test_learned_warm_start_real_grids(cfg, family):
+    """cfg4 (151x301x601 PFSS) and cfg5 (301x601x1201 'open'): the family's
+    network trained on 51x91x181 PCG solutions (committed weights, held-out
+    seed 0) predicts on 51x91x181, is prolonged to the configuration's grid
+    and starts the value path's solve (mixed precision, block-Jacobi ILU0):
+    the guess is kept, the answer is the zero guess's (phi, B, energy) and
+    the solve takes fewer iterations."""
     import os
     from p3d_driver import workloads
     from pot3d_amd import solver, warmstart as W
     wpath = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                         "pot3d_amd", "weights", "warmstart_pfss_51x91x181.npz")
+                         "pot3d_amd", "weights", f"warmstart_{family}_51x91x181.npz")
     if not os.path.exists(wpath):
-        pytest.skip("no trained weights at pot3d_amd/weights/ (python -m "
-                    "p3d_driver.train_warmstart --domain pfss writes them)")
-    w = workloads.config(4)
-    wc = workloads.domain_workload("pfss", workloads.COARSE_GRID, 0)
-    say(f"[gpu] cfg4 {w.name} N={w.N} warm start from {os.path.basename(wpath)}")
-    net, meta = W.WarmStartNet.load(wpath, wc.mesh)
+        pytest.skip(f"no trained weights at pot3d_amd/weights/ (python -m "
+                    f"p3d_driver.train_warmstart --domain {family} writes them)")
+    w = workloads.config(cfg)
+    wc = workloads.domain_workload(family, workloads.COARSE_GRID, 0)
+    say(f"[gpu] cfg{cfg} {w.name} N={w.N} warm start from {os.path.basename(wpath)}")
+    net, meta = W.WarmStartNet.load(wpath, wc.mesh, wc.pm_r1)
     with solver.Pot3dSolver(wc.mesh, wc.pm_r1, 2) as sc:
         sc.set_br0(wc.br0)
         phic = sc.fill_phi(net.predict(wc.br0))
@@ -227,4 +230,4 @@ def test_cfg4_learned_warm_start_real_grids():
         say(f"    zero guess {n0} iterations; learned guess (rel err {e:.3f}) {st['ncg']} "
             f"iterations, guess {st['guess']}, true relres {st['true_relres']:.2e}")
         assert st["guess"] == 1 and st["ncg"] < n0 and st["true_relres"] <= 1e-9
-        _check_fields(s, s.fill_phi(x), phi0, B0, w0, label="cfg4 learned vs zero guess")
+        _check_fields(s, s.fill_phi(x), phi0, B0, w0, label=f"cfg{cfg} learned vs zero guess")

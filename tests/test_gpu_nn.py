@@ -184,3 +184,60 @@ def test_training_lowers_loss_and_warm_start_gains():
         assert s.stats()["guess"] == 1
     net.close()
     s.close()
+
+
+def test_monopole_split():
+    """The monopole split of the warm start: x_m (the PCG solution for
+    br0 = 1) depends on r only; a constant added to br0 moves the prediction
+    by exactly that constant times x_m (the network sees the same map); and
+    on a grid small enough for the network's receptive field to span the
+    sphere (where the monopole is learnable without the split) the split
+    costs nothing in guess quality ('open' |Br| maps)."""
+    from pot3d_amd import _lib
+    from pot3d_amd import warmstart as W
+    from p3d_driver import workloads
+    from pot3d_amd.solver import Pot3dSolver
+    say("[gpu] monopole split of the warm start")
+
+    def wl(seed):
+        return workloads.pfss(21, 37, 72, seed, ifprec=2, n_bipoles=8, option="open", r0=2.5,
+                              r1=10.0)
+
+    w0 = wl(0)
+    mesh = w0.mesh
+    xm = W.monopole_response(mesh, w0.pm_r1)
+    X = xm.reshape(mesh.np - 2, mesh.nt - 2, mesh.nr - 2)
+    spread = float(np.max(np.abs(X - X[:1, :1, :])) / np.max(np.abs(X)))
+    say(f"    x_m: max deviation from its first column {spread:.2e}")
+    assert spread <= 1e-6
+    s = Pot3dSolver(mesh, w0.pm_r1, 2)
+    s.set_option(_lib.OPT_STOP, 1.0)
+    brs, xs, _ = _pairs(wl, range(100, 148), s)
+    means = [float(np.sum(W.area_weights(mesh) * b[1:-1, 1:-1]) / np.sum(W.area_weights(mesh)))
+             for b in brs]
+    say(f"    mean |Br| of the maps: {np.mean(means):.3e} (rms of the maps "
+        f"{np.mean([np.sqrt(np.mean(b ** 2)) for b in brs]):.3e})")
+    errs = {}
+    for mono in (0, 1):
+        net = W.WarmStartNet(mesh, seed=0)
+        if mono:
+            net.set_monopole(xm, W.area_weights(mesh))
+        W.train_network(net, brs, xs, epochs=24, lr=2e-3, seed=1)
+        e = []
+        for seed in (7, 8, 9, 10):
+            w = wl(seed)
+            s.set_br0(w.br0)
+            xz = np.zeros(s.N)
+            assert s.solve(xz, None, 1e-9)[0] == 0
+            e.append(float(np.linalg.norm(net.predict(w.br0) - xz) / np.linalg.norm(xz)))
+        errs[mono] = float(np.mean(e))
+        if mono:
+            b = wl(7).br0
+            d = net.predict(b + 0.5) - net.predict(b)
+            lin = float(np.max(np.abs(d - 0.5 * xm)) / np.max(np.abs(0.5 * xm)))
+            say(f"    prediction(br0 + 0.5) - prediction(br0) vs 0.5 x_m: {lin:.2e}")
+            assert lin <= 1e-3
+        net.close()
+    say(f"    held-out guess rel err without / with the split: {errs[0]:.3f} / {errs[1]:.3f}")
+    assert errs[1] <= 1.1 * errs[0] + 0.01
+    s.close()
