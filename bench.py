@@ -149,8 +149,11 @@ def measure_kernels(s, N: int, padded: int, reps: int = 20, graph: bool = True):
 VALUE_KERNELS = [
     ("ilu_apply_fp32_blockjacobi (k_tile_fwd_reg+k_tile_bwd_reg<float>)", 7, 24, 0),
     ("STENCIL_FP32", 1, 4, 4),
-    # p, z (tile-major, gathered), FP64 solution: 4+4+8 read, 4+8 written
-    ("p_update_fp32_fp64x (k_p_update<float>)", 8, 28, 0),
+    # per iteration of a group of 4: p, z (tile-major, gathered) read, p
+    # written (12 B) on every iteration, and on the group's last one also
+    # the FP64 solution read and written (16) and the group's 3 earlier
+    # directions read (12): (4*12 + 28) / 4 = 19 B
+    ("p_update_fp32_fp64x (k_p_update_x<float>, x every 4 iterations)", 8, 19, 0),
     # r, ap read, r written (tile-major)
     ("r_update_fp32 (k_r_update)", 9, 12, 0),
     ("rz_dot_fp32 (k_dot2<float>, tile order)", 10, 8, 0),

@@ -199,6 +199,18 @@ struct CgState {
                      // 3 non-finite residual, 4 zero right-hand side
   int stop_mode;     // 0: epsn (reference), 1: ||r||2/||b||2
   int x_ncg;         // iterations whose FP64 solution update is applied (FP32 recurrence)
+  // FP32 recurrence with the FP64 solution updated every kXBatch
+  // iterations: alpha of each iteration of the current group, and the
+  // buffer holding the current search direction
+  double alpha_hist[4];
+  int p_cur;
+};
+
+// Iterations per FP64 solution update of the FP32 recurrence (the p
+// update's group; directions of a group live in kXBatch rotating buffers).
+constexpr int kXBatch = 4;
+struct PBuf {
+  float* p[kXBatch];
 };
 
 // Control-state reads in kernels go through agent-scope atomic loads, so

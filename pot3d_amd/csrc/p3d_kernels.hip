@@ -778,6 +778,119 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The p update of the FP32 recurrence with the FP64 solution, the
+// solution updated once per group of kXBatch iterations: iteration t of a
+// group reads its direction p_t from pb.p[t] and writes p_(t+1) = z + beta
+// p_t to pb.p[(t+1) % kXBatch]; at the group's last iteration, or when the
+// recurrence has stopped, the solution takes the group's pending updates
+// x += (alpha_u scale) p_u in iteration order -- the arithmetic, in the
+// order, of one update per iteration (explicit fused multiply-adds), with
+// the solution read and written once instead of kXBatch times.  The
+// directions of earlier iterations are still in their buffers: p_(t+1)
+// overwrites p_(t+1-kXBatch), read first for the flush.
+template <int T_>
+__global__ void __launch_bounds__(kBlock)
+    k_p_update_x(Grid3 g, int G, int CH, PBuf pb, const float* __restrict__ z, CgState* st,
+                 TileMap tm, double* __restrict__ xs, const double* __restrict__ xscale,
+                 unsigned* counter) {
+  const int done = st_done(st);
+  const int ncg = st_load_int(&st->ncg);
+  const int pending = ncg - st_load_int(&st->x_ncg);  // iterations not in x yet
+  const bool flush = pending > 0 && (T_ == kXBatch - 1 || done);
+  if (pending > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    // this iteration's alpha for its group's flush, and where p now is
+    __hip_atomic_store(&st->alpha_hist[T_], st_load(&st->alpha), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st->p_cur, done ? T_ : (T_ + 1) % kXBatch, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (done && !flush) return;
+  const int Lb = logical_block(G);
+  const uint32_t n0 = (uint32_t)Lb * (uint32_t)CH;
+  const uint32_t n1 = min(g.N, n0 + (uint32_t)CH);
+  const float beta = (float)st_load(&st->beta);
+  const double sc = *xscale;
+  double ax[kXBatch];
+  bool use[kXBatch];
+#pragma unroll
+  for (int u = 0; u < kXBatch; u++) {
+    use[u] = flush && u <= T_ && u >= T_ + 1 - pending;
+    ax[u] = u == T_ ? st_load(&st->alpha) * sc : (use[u] ? st_load(&st->alpha_hist[u]) * sc : 0.0);
+  }
+  const float* __restrict__ pin = pb.p[T_];
+  float* __restrict__ pout = pb.p[(T_ + 1) % kXBatch];
+  constexpr int kU = 8;
+  for (uint32_t nb = n0 + threadIdx.x; nb < n1; nb += kU * kBlock) {
+    float pv[kU], zv[kU], pold[kXBatch][kU];
+    double xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const uint32_t n = nb + u * kBlock;
+      pv[u] = zv[u] = 0.f;
+      xv[u] = 0.0;
+#pragma unroll
+      for (int q = 0; q < kXBatch; q++) pold[q][u] = 0.f;
+      if (n < n1) {
+        pv[u] = __builtin_nontemporal_load(pin + n);
+        if (flush) {
+          xv[u] = __builtin_nontemporal_load(xs + n);
+#pragma unroll
+          for (int q = 0; q < T_; q++)
+            if (use[q]) pold[q][u] = __builtin_nontemporal_load(pb.p[q] + n);
+        }
+        if (!done) zv[u] = z[tmap(tm, g, n)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const uint32_t n = nb + u * kBlock;
+      if (n < n1) {
+        if (flush) {
+          double x = xv[u];
+#pragma unroll
+          for (int q = 0; q < T_; q++)
+            if (use[q]) x = __builtin_fma(ax[q], (double)pold[q][u], x);
+          x = __builtin_fma(ax[T_], (double)pv[u], x);
+          __builtin_nontemporal_store(x, xs + n);
+        }
+        if (!done) pout[n] = beta * pv[u] + zv[u];
+      }
+    }
+  }
+  if (flush) {
+    __shared__ int amlast;
+    __syncthreads();
+    // only the reads of x_ncg must precede its update: a plain count-in
+    if (threadIdx.x == 0)
+      amlast = (__hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                (unsigned)G - 1);
+    __syncthreads();
+    if (amlast && threadIdx.x == 0) {
+      __hip_atomic_store(&st->x_ncg, ncg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+void launch_p_update_x(const Grid3& g, const Launch& L, int t, const PBuf& pb, const float* z,
+                       CgState* st, hipStream_t s, const TileMap& tm, double* xs,
+                       const double* xscale, unsigned* counter) {
+  switch (t) {
+    case 0:
+      k_p_update_x<0><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      break;
+    case 1:
+      k_p_update_x<1><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      break;
+    case 2:
+      k_p_update_x<2><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      break;
+    default:
+      k_p_update_x<3><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      break;
+  }
+}
+
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
                      const T* r, const T* z, CgState* st, hipStream_t s, const TileMap& tm,

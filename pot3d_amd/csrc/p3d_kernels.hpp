@@ -138,6 +138,13 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
                       hipStream_t s, double* xs = nullptr, const double* xscale = nullptr,
                       const TileMap& tm = TileMap{});
 
+// p update of the FP32 recurrence at iteration t of a group of kXBatch,
+// with the group's FP64 solution updates at its last iteration (or when the
+// recurrence has stopped): k_p_update_x in p3d_kernels.hip.
+void launch_p_update_x(const Grid3& g, const Launch& L, int t, const PBuf& pb, const float* z,
+                       CgState* st, hipStream_t s, const TileMap& tm, double* xs,
+                       const double* xscale, unsigned* counter);
+
 // p = z + beta p (z = M^-1 r computed in place for DIAG/NONE, read for ILU).
 // tm: placement of r (cg_update) and z (p_update) (natural by default).
 // With xs (FP32 recurrence, FP64 solution) it first applies the pending

@@ -56,6 +56,9 @@ struct Vecs {
   T *x, *r, *p, *ap, *z;
   double* xs = nullptr;             // FP64 solution updated instead of x (FP32 recurrence)
   const double* xscale = nullptr;  // scale of the FP32 system (device)
+  // FP32 recurrence with the FP64 solution on tile-major r, ap, z: the
+  // kXBatch rotating direction buffers (pb.p[0] = p), x updated per group
+  PBuf pb{};
 };
 
 struct p3d_solver {
@@ -78,6 +81,7 @@ struct p3d_solver {
   float* yv = nullptr;
   float* dinv = nullptr;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr;
+  float* p32b[kXBatch - 1] = {};  // the other direction buffers of the grouped x updates
   // r, z (and r32, z32) hold nvec >= N entries: with the single-wave tile
   // sweeps they are tile-major (tm.on), every other vector is natural
   size_t nvec = 0;
@@ -145,6 +149,16 @@ static int ensure_mixed(p3d_solver* s) {
   HIPCHK(hipMemsetAsync(s->r32, 0, nv, s->stream));
   HIPCHK(hipMemsetAsync(s->z32, 0, nv, s->stream));
   HIPCHK(hipMemsetAsync(s->ap32, 0, nv, s->stream));
+  return P3D_OK;
+}
+
+// The other direction buffers of the FP32 recurrence's grouped solution
+// updates (natural order, N floats each).
+static int ensure_pbuf(p3d_solver* s) {
+  int rc;
+  for (int q = 0; q < kXBatch - 1; q++)
+    if (!s->p32b[q] && (rc = dalloc((void**)&s->p32b[q], (size_t)s->g.N * sizeof(float))))
+      return rc;
   return P3D_OK;
 }
 
@@ -232,20 +246,23 @@ static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, 
   }
 }
 
+// One CG iteration; t is its place in the group of kXBatch iterations of
+// the FP32 recurrence's grouped solution updates (unused otherwise).
 template <typename T>
-static void cg_iteration(p3d_solver* s, Vecs<T> v) {
+static void cg_iteration(p3d_solver* s, Vecs<T> v, int t) {
   if constexpr (sizeof(T) == 4) {
-    if (v.xs && s->tm.on) {
+    if (v.xs && s->tm.on && v.pb.p[0]) {
       // FP32 recurrence with the FP64 solution and tile-major r, ap, z:
-      // every pass contiguous except the p update's read of z
-      launch_pole_sums<float>(s->m, s->g, v.p, s->pole_part, s->stream);
-      launch_stencil<float>(s->m, s->g, s->L, v.p, v.ap, nullptr, 0, s->pole_part, s->st, s->red,
+      // every pass contiguous except the p update's read of z; direction
+      // p_t in buffer t of the group, the FP64 x updated once per group
+      float* p = v.pb.p[t];
+      launch_pole_sums<float>(s->m, s->g, p, s->pole_part, s->stream);
+      launch_stencil<float>(s->m, s->g, s->L, p, v.ap, nullptr, 0, s->pole_part, s->st, s->red,
                             0, EPI_PAP, nullptr, true, s->stream, s->tm);
       launch_r_update(s->gt.N, s->Lt, v.r, v.ap, s->st, s->red, 2, s->stream);
       prec_and_dot<float>(s, v.r, v.z, EPI_RZITER, nullptr, true);
-      // the p update also applies this iteration's x += alpha scale p_old
-      launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm,
-                             v.xs, v.xscale, s->red.counter + 5 * kRedCounters);
+      launch_p_update_x(s->g, s->L, t, v.pb, v.z, s->st, s->stream, s->tm, v.xs, v.xscale,
+                        s->red.counter + 5 * kRedCounters);
       return;
     }
   }
@@ -266,7 +283,9 @@ static int chunk_size(p3d_solver* s) {
                                                            : 2 * (s->g.a + s->g.b + s->g.c);
     int k = 4096 / per;
     if (k > 32) k = 32;
-    return k < 1 ? 1 : k;
+    // whole groups of the FP32 recurrence's grouped solution updates
+    if (sizeof(T) == 4) k = k / kXBatch * kXBatch;
+    return k < kXBatch ? kXBatch : k;
   }
   return 32;
 }
@@ -280,7 +299,7 @@ static int run_iterations(p3d_solver* s, Vecs<T> v, int K) {
       s->gexec[ti] = nullptr;
       hipGraph_t graph;
       HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < K; it++) cg_iteration<T>(s, v);
+      for (int it = 0; it < K; it++) cg_iteration<T>(s, v, it % kXBatch);
       HIPCHK(hipStreamEndCapture(s->stream, &graph));
       hipError_t e = hipGraphInstantiate(&s->gexec[ti], graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
@@ -289,7 +308,7 @@ static int run_iterations(p3d_solver* s, Vecs<T> v, int K) {
     }
     HIPCHK(hipGraphLaunch(s->gexec[ti], s->stream));
   } else {
-    for (int it = 0; it < K; it++) cg_iteration<T>(s, v);
+    for (int it = 0; it < K; it++) cg_iteration<T>(s, v, it % kXBatch);
     HIPCHK(hipGetLastError());
   }
   return P3D_OK;
@@ -487,8 +506,14 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
   s->out_host[5] = 1.0;
   HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, 2 * sizeof(double), hipMemcpyHostToDevice,
                         s->stream));
-  // the recurrence's solution updates go straight to the FP64 x
+  // the recurrence's solution updates go straight to the FP64 x, grouped
+  // kXBatch iterations at a time with the tile-major placement
   Vecs<float> v{s->x32, s->r32, s->p32, s->ap32, s->z32, s->x, s->out + 4};
+  if (s->tm.on) {
+    if ((rc = ensure_pbuf(s))) return rc;
+    v.pb.p[0] = s->p32;
+    for (int q = 1; q < kXBatch; q++) v.pb.p[q] = s->p32b[q - 1];
+  }
   // the residual update runs over every entry of the tile-major r and ap:
   // entries outside the grid must be zero in both (other users of the
   // buffers may have written them)
@@ -549,6 +574,14 @@ static int solve_mixed_ru(p3d_solver* s, double epscg, int ncgmax) {
     HIPCHK(hipMemcpyAsync(s->st, s->st_host, sizeof(CgState), hipMemcpyHostToDevice,
                           s->stream));
     prec_and_dot<float>(s, v.r, v.z, EPI_REPLACE, nullptr, false);
+    // the recurrence's direction to buffer 0, where the next group starts
+    const int pc = s->st_host->p_cur;
+    if (v.pb.p[0] && pc > 0 && pc < kXBatch) {
+      launch_copy<float>(s->g, s->L, v.p, v.pb.p[pc], s->stream);
+      s->st_host->p_cur = 0;
+      HIPCHK(hipMemcpyAsync(&s->st->p_cur, &s->st_host->p_cur, sizeof(int),
+                            hipMemcpyHostToDevice, s->stream));
+    }
     launch_p_update<float>(s->m, s->g, s->L, s->ifprec, v.p, v.r, v.z, s->st, s->stream, s->tm);
     if ((rc = read_state(s))) return rc;
   }
@@ -669,6 +702,8 @@ int p3d_destroy(p3d_solver* s) {
                   s->st, s->hist, s->red.partial, s->red.counter, s->out, s->phi, s->sums,
                   s->e_partial};
   for (void* b : bufs)
+    if (b) hipFree(b);
+  for (float* b : s->p32b)
     if (b) hipFree(b);
   free_sweep_plan(s->sweep);
   if (s->st_host) hipHostFree(s->st_host);
@@ -1252,6 +1287,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
     return fail(P3D_ESTATE, "no tile-major placement (single-wave ILU tile sweeps)");
   if (which == 1 || which == 3 || which >= 7) {
     int rc = ensure_mixed(s);
+    if (!rc && which == 8) rc = ensure_pbuf(s);
     if (rc) return rc;
   }
   hipEvent_t e0, e1;
@@ -1264,7 +1300,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   init.bb = 1.0;
   init.epscg = 0.0;
   init.ncgmax = 1 << 30;
-  init.ncg = 1;
+  init.ncg = which == 8 ? kXBatch : 1;
   *s->st_host = init;
   s->out_host[4] = 1.0;
   HIPCHK(hipMemcpyAsync(s->out + 4, s->out_host + 4, sizeof(double), hipMemcpyHostToDevice,
@@ -1324,13 +1360,19 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
       case 7:  // FP32 ILU apply (the mixed-precision inner preconditioner)
         if (s->ifprec == PREC_ILU) prec_and_dot<float>(s, s->r32, s->z32, EPI_NONE, nullptr, false);
         break;
-      case 8:  // p update of the FP32 recurrence with the FP64 solution update
-        // (x_ncg cleared so that every launch applies it; alpha = 0 leaves x)
+      case 8: {  // one group of the FP32 recurrence's p updates (kXBatch launches, the
+        // last with the FP64 solution update of the whole group; x_ncg cleared
+        // so that it applies kXBatch pending updates; alpha = 0 leaves x);
+        // timed per iteration
+        PBuf pb{};
+        pb.p[0] = s->p32;
+        for (int q = 1; q < kXBatch; q++) pb.p[q] = s->p32b[q - 1];
         (void)hipMemsetAsync(&s->st->x_ncg, 0, sizeof(int), s->stream);  // checked below
-        launch_p_update<float>(s->m, s->g, s->L, s->ifprec, s->p32, s->r32, s->z32, s->st,
-                               s->stream, s->tm, s->x, s->out + 4,
-                               s->red.counter + 5 * kRedCounters);
+        for (int t = 0; t < kXBatch; t++)
+          launch_p_update_x(s->g, s->L, t, pb, s->z32, s->st, s->stream, s->tm, s->x,
+                            s->out + 4, s->red.counter + 5 * kRedCounters);
         break;
+      }
       case 9:  // residual update of the FP32 recurrence (tile-major r, ap)
         launch_r_update(s->gt.N, s->Lt, s->r32, s->ap32, s->st, s->red, 2, s->stream);
         break;
@@ -1352,7 +1394,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
-  *avg_ms = (double)ms / reps;
+  *avg_ms = (double)ms / reps / (which == 8 ? kXBatch : 1);
   if (which > 10 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
   return P3D_OK;
 }
