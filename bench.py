@@ -117,8 +117,8 @@ def cpu_baseline(workload, family: str, cores: int, sample_iters: int, n_iter_es
 # (name, time_kernel id, algorithmic bytes per unknown, padded-array bytes per point)
 KERNELS = [
     ("pcg_iteration_fp64 (graph replay)", 3, 136, 8),
-    ("stencil_fp64 (k_stencil_march<double>)", 0, 8, 8),
-    ("STENCIL_FP32", 1, 4, 4),
+    ("stencil_fp64 (k_stencil_lds<double>)", 0, 8, 8),
+    ("stencil_fp32 (k_stencil_lds<float>)", 1, 4, 4),
     ("ilu_apply_fp64 (k_tile_fwd+k_tile_bwd<double>)", 2, 32, 0),
     ("ilu_apply_fp32 (k_tile_fwd+k_tile_bwd<float>)", 7, 24, 0),
     ("cg_update_fp64 (k_cg_update<double>)", 4, 48, 0),
@@ -135,8 +135,6 @@ def measure_kernels(s, N: int, padded: int, nr: int, reps: int = 20, graph: bool
     for name, kid, per_n, per_pad in KERNELS:
         if kid == 3 and not graph:
             continue
-        if name == "STENCIL_FP32":
-            name = stencil_fp32_kernel(nr)
         ms = s.time_kernel(kid, reps if kid not in (2, 7) else 5)
         nbytes = per_n * N + per_pad * padded
         table[name] = {"ms": round(ms, 5), "bytes": nbytes,
@@ -146,11 +144,10 @@ def measure_kernels(s, N: int, padded: int, nr: int, reps: int = 20, graph: bool
 
 # Kernels of the value path's iteration: FP32 recurrence with the
 # block-Jacobi ILU0 on register tiles.  (name, time_kernel id, algorithmic
-# bytes per unknown, padded-array bytes per point); the FP32 operator's
-# kernel is the library's automatic choice for the grid (STENCIL_FP32).
+# bytes per unknown, padded-array bytes per point).
 VALUE_KERNELS = [
     ("ilu_apply_fp32_blockjacobi (k_tile_fwd_reg+k_tile_bwd_reg<float>)", 7, 24, 0),
-    ("STENCIL_FP32", 1, 4, 4),
+    ("stencil_fp32 (k_stencil_lds<float>)", 1, 4, 4),
     # per iteration of a group of 4: p, z (tile-major, gathered) read, p
     # written (12 B) on every iteration, and on the group's last one also
     # the FP64 solution read and written (16) and the group's 3 earlier
@@ -175,24 +172,12 @@ def set_fast_ilu(s, args):
     return blocks
 
 
-def stencil_fp32_kernel(nr: int) -> str:
-    """The FP32 operator kernel the library chooses by default (P3D_OPT_STENCIL
-    = -1): the row-blocked march when its 62-lane waves cover the nr-2 radial
-    points to at least 90%, else the column march."""
-    a = nr - 2
-    nwi = -(-a // 62)
-    rows = 10 * a >= 9 * nwi * 62
-    return "stencil_fp32 (k_stencil_rows<float>)" if rows else "stencil_fp32 (k_stencil_march<float>)"
-
-
 def measure_value_kernels(s, N: int, padded: int, nr: int, reps: int = 5):
     """The value path's kernels (set_fast_ilu applied), timed as
     measure_kernels; the ILU apply is the dominant one, so it is the
     roofline kernel.  Also the algorithmic bytes of one iteration."""
     table = {}
     for name, kid, per_n, per_pad in VALUE_KERNELS:
-        if name == "STENCIL_FP32":
-            name = stencil_fp32_kernel(nr)
         ms = s.time_kernel(kid, reps)
         nbytes = per_n * N + per_pad * padded
         table[name] = {"ms": round(ms, 5), "bytes": nbytes,
@@ -433,7 +418,7 @@ def main():
                  "achieved": round(vkern["iter_bytes"] / (ms_iter * 1e-3) / 1e9, 1),
                  "frac": round(vkern["iter_bytes"] / (ms_iter * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                  "kernels_ms_sum": round(vkern["kernels_ms"], 4)}
-    st_name = "stencil_fp64 (k_stencil_march<double>)"
+    st_name = "stencil_fp64 (k_stencil_lds<double>)"
     per_iter64 = min(t64) / max(st64["ncg"], 1)
     out = {
         "metric": "PCG time-to-||r||/||b||<=1e-9 (s): mixed+learned vs FP64 plain; "

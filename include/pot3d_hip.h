@@ -131,10 +131,16 @@ enum {
   P3D_OPT_STENCIL = 15,     /* operator kernel: 0 one thread per (i,j)
                                 column marching phi; 1 each thread holds 8
                                 theta rows of one column, the neighbours in
-                                i from the adjacent lanes; -1 (default) 1 for
-                                the FP32 operator when its 62-lane waves
-                                cover the r extent to 90% or more, else 0.
-                                Same operator values bit for bit. */
+                                i from the adjacent lanes; 2 64 x 4WJ (r,
+                                theta) patches staged plane by plane in LDS,
+                                WJ = 8 when that still makes 2048 or more
+                                workgroups, else 4; 3 and 4 the same with
+                                WJ = 8 and WJ = 4; -1 (default) 2.  The LDS
+                                kernels apply A x; b - A x and outputs of
+                                2^32 bytes or more take 1 (FP32 operator
+                                whose 62-lane waves cover the r extent to
+                                90% or more) or 0.  Same operator values
+                                bit for bit. */
   P3D_OPT_APPLY_FP32 = 16   /* p3d_apply_A: 1 applies the FP32 operator of
                                 the mixed-precision recurrence (x rounded to
                                 float, the result in float); 0 FP64 */

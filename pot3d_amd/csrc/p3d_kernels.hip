@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "p3d_kernels.hpp"
 
@@ -517,6 +518,322 @@ __global__ void __launch_bounds__(kBlock)
   block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged operator march (Launch::stencil = 2; y = A x): the operator and
+// the per-point arithmetic of k_stencil_march, with every value of x loaded
+// from memory once per workgroup.  A workgroup of four waves owns a patch of
+// kLdsW = 64 consecutive i by 4 WJ theta rows and marches a segment of phi
+// planes.  Per plane, the patch and its one-point halo ((4 WJ + 2) rows of
+// 66 values) are loaded D planes ahead into registers by coalesced loads,
+// then written to one of D LDS slots; wave w reads rows w WJ .. w WJ + WJ - 1
+// of the slot at column i0 + lane and their neighbours, and keeps the
+// centres of planes k-1 and k in registers.  One barrier per plane orders
+// the slot writes of plane k+1 before their reads; with D >= 3 slots the
+// slot written at plane k was last read at plane k-2, which every wave
+// finished before that barrier.  The barrier waits for LDS only, so the
+// loads of the next planes stay in flight across it.  Patches are ordered r
+// fastest: neighbouring patches share the cache lines at their edges (their
+// halo columns, and the lines of y that both write part of), and they run
+// together on one L2.
+//
+// WJ: theta rows per wave; D: planes loaded ahead (= LDS slots = unroll of
+// the march); WPE: waves per SIMD the register allocation is held to.  The
+// tall patches (WJ = 8) load fewer halo rows per point, the short ones
+// (WJ = 4) make twice the workgroups for the smaller grids.
+template <int WJ_, int D_, int WPE_>
+struct LdsShape {
+  static constexpr int WJ = WJ_, D = D_, WPE = WPE_;
+};
+template <typename T, bool kTall>
+struct LdsCfg;
+template <> struct LdsCfg<double, true> : LdsShape<8, 3, 2> {};
+template <> struct LdsCfg<double, false> : LdsShape<4, 3, 1> {};
+template <> struct LdsCfg<float, true> : LdsShape<8, 4, 2> {};
+template <> struct LdsCfg<float, false> : LdsShape<4, 3, 4> {};
+constexpr int kLdsW = 64;
+constexpr int kLdsRow = kLdsW + 2;
+constexpr int kLdsMaxPlanes = 96;  // phi planes of a segment (per-plane factors in LDS)
+
+template <typename C>
+__host__ __device__ inline void stencil_lds_shape(const Grid3& g, int& nwi, int& nwj) {
+  nwi = (g.a + kLdsW - 1) / kLdsW;
+  nwj = (g.b + 4 * C::WJ - 1) / (4 * C::WJ);
+}
+
+// Raw buffer stores of one value: a lane whose byte offset is kDropStore
+// writes nothing (out of the buffer's range), so a store needs no branch.
+constexpr uint32_t kDropStore = 0xffffffffu;
+__device__ __forceinline__ void store_or_drop(__amdgpu_buffer_rsrc_t rs, uint32_t off,
+                                              double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  u32x2 d;
+  d.x = (unsigned)(u & 0xffffffffull);
+  d.y = (unsigned)(u >> 32);
+  __builtin_amdgcn_raw_buffer_store_b64(d, rs, (int)off, 0, 0);
+}
+__device__ __forceinline__ void store_or_drop(__amdgpu_buffer_rsrc_t rs, uint32_t off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32((unsigned)__float_as_uint(v), rs, (int)off, 0, 0);
+}
+
+template <typename T, typename C>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(C::WPE)))
+    k_stencil_lds(Metric m, Grid3 g, int nseg, const T* __restrict__ x, T* __restrict__ y,
+                  const double* __restrict__ pole_part, CgState* st, double* partial,
+                  unsigned* counter, Epi epi, double* out, int guard, int want_dot, TileMap tm) {
+  constexpr int WJ = C::WJ, PJ = 4 * WJ, D = C::D;
+  constexpr int PE = (PJ + 2) * kLdsRow;          // values of a plane slot
+  constexpr int NL = (PE + kBlock - 1) / kBlock;  // loads per thread per plane
+  // The march is straight-line code: no branch around a load or a store.  The
+  // compiler's wait for a plane's loads counts the memory operations issued
+  // after them on every path, so a skipped store or load would make it wait
+  // for the planes behind as well.  Hence the slots are padded to NL kBlock
+  // values (every thread loads and stages NL values; the padding receives
+  // copies of the last one), the lookahead past the segment reloads its last
+  // plane, and out-of-grid points are stored to an out-of-range offset.
+  __shared__ T S[D][NL * kBlock];
+  __shared__ double F[3][kLdsMaxPlanes];
+  if (guard && st_done(st)) return;
+  const int G = gridDim.x;
+  const int Lb = logical_block(G);
+  int nwi, nwj;
+  stencil_lds_shape<C>(g, nwi, nwj);
+  const int seg = Lb / (nwi * nwj), rem = Lb - seg * (nwi * nwj);
+  const int wj = rem / nwi, wi = rem - wj * nwi;
+  const int kper = (g.c + nseg - 1) / nseg;
+  const int k0 = seg * kper, k1 = min(g.c, k0 + kper);
+  const int tid = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int i0 = wi * kLdsW, j0 = wj * PJ;
+  double acc = 0.0;
+  // workgroup-uniform: every thread takes part in the loads and barriers
+  if (seg < nseg && k0 < k1) {
+    const int n = k1 - k0;  // planes k0 .. k1-1; x planes k0-1 .. k1 are read
+    const uint32_t ab = g.ab, a_ = (uint32_t)g.a;
+    const int jw = j0 + wave * WJ;  // the wave's first row (wave-uniform)
+    // factors of the wave's rows, loaded before any store or barrier (so as
+    // scalar loads), and the segment's per-plane factors staged in LDS: the
+    // march issues no vector load but the x planes
+    double SDr[WJ], CTLr[WJ], CTHr[WJ], CPr[WJ];
+#pragma unroll
+    for (int r = 0; r < WJ; r++) {
+      const int j = min(jw + r, g.b - 1);
+      SDr[r] = m.SD[j];
+      CTLr[r] = m.CTL[j];
+      CTHr[r] = m.CTH[j];
+      CPr[r] = m.CP[j];
+    }
+    for (int q = tid; q < n; q += kBlock) {
+      F[0][q] = m.DPH[k0 + q];
+      F[1][q] = m.DPL[k0 + q];
+      F[2][q] = m.DPU[k0 + q];
+    }
+    // the thread's share of a slot: values e = tid + kBlock q, row e / (kLdsW
+    // + 2) of the patch (theta j0-1+row), column e % (kLdsW + 2) (r i0-1+col),
+    // clamped into the grid (clamped values are never used)
+    uint32_t off[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int e = min(tid + kBlock * q, PE - 1);
+      const int r = e / kLdsRow, c = e - r * kLdsRow;
+      const int ii = min(max(i0 - 1 + c, 0), g.a - 1);
+      const int jj = min(max(j0 - 1 + r, 0), g.b - 1);
+      off[q] = (uint32_t)ii + a_ * (uint32_t)jj;
+    }
+    const int i = i0 + lane;
+    const bool col_ok = i < g.a;
+    const int il = min(i, g.a - 1);
+    const double DRH = m.DRH[il], CRL = m.CRL[il], CRH = m.CRH[il];
+    const bool im = i > 0, ip = i < g.a - 1;
+    const double s0d = jw == 0 ? 2.0 * pole_sum(pole_part, 0, il, g.a) : 0.0;
+    const double s1d = jw + WJ >= g.b ? 2.0 * pole_sum(pole_part, 1, il, g.a) : 0.0;
+    const __amdgpu_buffer_rsrc_t ys =
+        __builtin_amdgcn_make_buffer_rsrc(y, 0, (int)kDropStore, 0x00020000);
+    // output offsets (in values) of the wave's rows at plane 0, natural or
+    // tile-major (8x8 (i, j) tile planes of 64 values, 2^tk_log2 planes per
+    // tile), and the per-plane step; out-of-grid points are dropped
+    uint32_t ybase[WJ];
+    bool yok[WJ];
+#pragma unroll
+    for (int r = 0; r < WJ; r++) {
+      const int j = min(jw + r, g.b - 1);
+      yok[r] = col_ok && jw + r < g.b;
+      ybase[r] = tm.on ? ((((uint32_t)(il >> 3) + (uint32_t)tm.TNI * (uint32_t)(j >> 3))
+                           << (tm.tk_log2 + 6)) +
+                          (uint32_t)(((j & 7) << 3) | (il & 7)))
+                       : (uint32_t)il + a_ * (uint32_t)j;
+    }
+    const uint32_t tplane = (uint32_t)tm.TNI * (uint32_t)tm.TNJ;
+    auto yplane = [&](int k) -> uint32_t {
+      return tm.on ? (((tplane * ((uint32_t)k >> tm.tk_log2)) << (tm.tk_log2 + 6)) +
+                      (((uint32_t)k & ((1u << tm.tk_log2) - 1u)) << 6))
+                   : ab * (uint32_t)k;
+    };
+    // the wave's rows in a slot: row 1 + wave WJ + r, column lane + 1
+    const int rbase = (1 + wave * WJ) * kLdsRow + lane + 1;
+    T B[D][NL];
+    T prv[WJ], cur[WJ];
+    // x plane p of the segment (p = 0 .. n+1 is phi plane k0-1+p, periodic);
+    // the lookahead past plane n+1 reloads plane n+1 (cached, never staged)
+    auto load = [&](T (&R)[NL], int p) {
+      int kw = k0 - 1 + min(p, n + 1);
+      kw = kw < 0 ? kw + g.c : (kw >= g.c ? kw - g.c : kw);
+      const T* xp = x + (size_t)ab * (uint32_t)kw;
+#pragma unroll
+      for (int q = 0; q < NL; q++) R[q] = xp[off[q]];
+      // keep the loads where they are in the sequence of memory operations
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    // step p: load plane p+D into buffer p%D, stage plane p+1 (buffer and
+    // slot (p+1)%D), barrier; for p >= 1 compute plane p (slot p%D) with
+    // the centres of plane p+1
+    auto step = [&](auto pm_c, auto mode_c, int p) {
+      constexpr int PM = decltype(pm_c)::value, PN = (PM + 1) % D;
+      constexpr int MODE = decltype(mode_c)::value;  // 0 prologue, 1 first centres, 2 plane
+      load(B[PM], p + D);
+#pragma unroll
+      for (int q = 0; q < NL; q++) S[PN][tid + kBlock * q] = B[PN][q];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const T* Sc = S[PM];
+      const T* Sn = S[PN];
+      if constexpr (MODE != 2) {
+        // the prologue steps issue the plane's WJ stores too (dropped), so
+        // the memory operations between a plane's loads and its staging are
+        // the same on entry to the march as in it, and so are the waits
+#pragma unroll
+        for (int r = 0; r < WJ; r++) {
+          uint32_t yo = kDropStore;
+          asm volatile("" : "+v"(yo));
+          store_or_drop(ys, yo, (T)0);
+        }
+      }
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int r = 0; r < WJ; r++) {
+          prv[r] = Sc[rbase + r * kLdsRow];
+          cur[r] = Sn[rbase + r * kLdsRow];
+        }
+      } else if constexpr (MODE == 2) {
+        T nxt[WJ], xw[WJ], xe[WJ];
+#pragma unroll
+        for (int r = 0; r < WJ; r++) {
+          nxt[r] = Sn[rbase + r * kLdsRow];
+          xw[r] = Sc[rbase + r * kLdsRow - 1];
+          xe[r] = Sc[rbase + r * kLdsRow + 1];
+        }
+        const T xup = Sc[rbase - kLdsRow];
+        const T xdn = Sc[rbase + WJ * kLdsRow];
+        const int k = k0 - 1 + p;
+        const double dph = F[0][p - 1], dpl = F[1][p - 1], dpu = F[2][p - 1];
+        const uint32_t yk = yplane(k);
+#pragma unroll
+        for (int r = 0; r < WJ; r++) {
+          const int j = jw + r;
+          const bool jm = j > 0, jp = j < g.b - 1;
+          const double P1 = DRH * CPr[r];
+          double co[7];
+          op_coefs(DRH, CRL, CRH, SDr[r], CTLr[r], CTHr[r], P1, dph, dpl, dpu, co);
+          const T xs = r == 0 ? xup : cur[r - 1];
+          const T xn = r == WJ - 1 ? xdn : cur[r + 1];
+          const T yv = op_point<T>(co, prv[r], xs, xw[r], cur[r], xe[r], xn, nxt[r], im, ip, jm,
+                                   jp, m.pm_r1, s0d, s1d, false, (T)0);
+          uint32_t yo = yok[r] ? (ybase[r] + yk) * (uint32_t)sizeof(T) : kDropStore;
+          // an opaque offset: the store stays one unconditional instruction
+          asm volatile("" : "+v"(yo));
+          store_or_drop(ys, yo, yv);
+          acc += yok[r] ? (double)cur[r] * (double)yv : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < WJ; r++) {
+          prv[r] = cur[r];
+          cur[r] = nxt[r];
+        }
+      }
+    };
+    using C0 = std::integral_constant<int, 0>;
+    using C1 = std::integral_constant<int, 1>;
+    using C2 = std::integral_constant<int, 2>;
+    using C3 = std::integral_constant<int, 3>;
+    using Cm = std::integral_constant<int, D - 1>;
+    using Cp2 = std::integral_constant<int, 2 % D>;
+    using Cp3 = std::integral_constant<int, 3 % D>;
+    // prologue as steps -1 and 0: planes 0 and 1 staged, the centres of
+    // planes 0 and 1 read, planes 2 .. D in flight
+#pragma unroll
+    for (int q = 0; q + 1 < D; q++) load(B[q], q);
+    step(Cm{}, C0{}, -1);
+    step(C0{}, C1{}, 0);
+    // whole rounds of D planes, then the remaining ones: a loop with one
+    // entry and one exit, entered in the state it repeats
+    const int nfull = n / D, nrem = n - nfull * D;
+    for (int t = 0; t < nfull; t++) {
+      const int p = 1 + D * t;
+      step(C1{}, C2{}, p);
+      step(Cp2{}, C2{}, p + 1);
+      step(Cp3{}, C2{}, p + 2);
+      if constexpr (D == 4) step(C0{}, C2{}, p + 3);
+    }
+    const int pr = 1 + D * nfull;
+    if (nrem >= 1) step(C1{}, C2{}, pr);
+    if (nrem >= 2) step(Cp2{}, C2{}, pr + 1);
+    if constexpr (D == 4)
+      if (nrem >= 3) step(C3{}, C2{}, pr + 2);
+  }
+  if (!want_dot) return;
+  double v[1] = {acc};
+  block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
+}
+
+template <typename C>
+static int stencil_lds_segments(const Grid3& g) {
+  // about 4 rounds of 4 workgroups per CU, >= kSegPlanes planes each
+  int nwi, nwj;
+  stencil_lds_shape<C>(g, nwi, nwj);
+  int nseg = (4 * 4 * 256 + nwi * nwj - 1) / (nwi * nwj);
+  const int maxseg = (g.c + kSegPlanes - 1) / kSegPlanes;
+  if (nseg > maxseg) nseg = maxseg;
+  // segments of at most kLdsMaxPlanes planes
+  nseg = std::max(nseg, (g.c + kLdsMaxPlanes - 1) / kLdsMaxPlanes);
+  if (nseg < 1) nseg = 1;
+  return nseg;
+}
+
+template <typename C>
+static int stencil_lds_blocks(const Grid3& g) {
+  int nwi, nwj;
+  stencil_lds_shape<C>(g, nwi, nwj);
+  const int G = nwi * nwj * stencil_lds_segments<C>(g);
+  return (G + 7) / 8 * 8;
+}
+
+static int stencil_lds_blocks_max(const Grid3& g) {
+  return std::max({stencil_lds_blocks<LdsCfg<double, true>>(g),
+                   stencil_lds_blocks<LdsCfg<double, false>>(g),
+                   stencil_lds_blocks<LdsCfg<float, true>>(g),
+                   stencil_lds_blocks<LdsCfg<float, false>>(g)});
+}
+
+// The tall patches when they still make about 8 workgroups per CU.
+template <typename T>
+static bool stencil_lds_tall(const Grid3& g) {
+  return stencil_lds_blocks<LdsCfg<T, true>>(g) >= 8 * 256;
+}
+
+template <typename T, typename C>
+static void launch_stencil_lds(const Metric& m, const Grid3& g, const T* x, T* y,
+                               const double* pole_part, CgState* st, Reduce red, int slot,
+                               Epi epi, double* out, bool guard, int want, hipStream_t s,
+                               const TileMap& tm) {
+  const int nseg = stencil_lds_segments<C>(g);
+  const int G = stencil_lds_blocks<C>(g);
+  k_stencil_lds<T, C><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, pole_part, st, red.partial,
+                                            red.counter + (size_t)slot * kRedCounters, epi, out,
+                                            guard ? 1 : 0, want, tm);
+}
+
 static int stencil_rows_segments(const Grid3& g) {
   // about 16 workgroups (64 waves) per CU in total, >= kSegPlanes planes each
   int nwi, nwg;
@@ -547,7 +864,7 @@ static int stencil_segments(const Grid3& g, const TileMap& tm) {
 
 int stencil_blocks(const Grid3& g, const TileMap& tm) {
   const int G = stencil_col_blocks(g, tm) * stencil_segments(g, tm);
-  return std::max((G + 7) / 8 * 8, stencil_rows_blocks(g));
+  return std::max({(G + 7) / 8 * 8, stencil_rows_blocks(g), stencil_lds_blocks_max(g)});
 }
 
 template <typename T>
@@ -556,6 +873,20 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     int slot, Epi epi, double* out, bool guard, hipStream_t s,
                     const TileMap& tm) {
   const int want = (epi != EPI_NONE);
+  // the LDS-staged march (y = A x only) stores through 32-bit buffer offsets
+  // (the output, tile padding included, is below 2 N values)
+  const bool lds_ok = mode == 0 && 2 * (double)g.N * sizeof(T) < (double)kDropStore;
+  if (lds_ok && (L.stencil < 0 || L.stencil >= 2)) {
+    const bool tall = L.stencil == 3 || ((L.stencil < 0 || L.stencil == 2) &&
+                                         stencil_lds_tall<T>(g));
+    if (tall)
+      launch_stencil_lds<T, LdsCfg<T, true>>(m, g, x, y, pole_part, st, red, slot, epi, out,
+                                             guard, want, s, tm);
+    else
+      launch_stencil_lds<T, LdsCfg<T, false>>(m, g, x, y, pole_part, st, red, slot, epi, out,
+                                              guard, want, s, tm);
+    return;
+  }
   if (L.stencil == 1 || (L.stencil < 0 && stencil_rows_chosen(g, sizeof(T)))) {
     const int nseg = stencil_rows_segments(g);
     const int G = stencil_rows_blocks(g);

@@ -89,8 +89,8 @@ __device__ inline void cg_epilogue(CgState* st, Epi epi, double s0, double s1, d
 struct Launch {
   int G;        // blocks of a streaming kernel (multiple of 8)
   int CH;       // points per block (multiple of 256)
-  // operator kernel: 0 column march, 1 row-blocked march, -1 chosen per
-  // precision and grid (stencil_rows_chosen)
+  // operator kernel: 0 column march, 1 row-blocked march, 2 LDS-staged
+  // patches (3 tall, 4 short), -1 the LDS-staged patches where they apply
   int stencil = -1;
 };
 

@@ -124,8 +124,12 @@ struct p3d_solver {
 };
 
 // ---------------------------------------------------------------------------
+// Every device buffer gets kTailPad bytes past its end: the 16-byte vector
+// loads of the LDS-staged operator (k_stencil_lds) read whole aligned vectors,
+// the last of which may reach past the last value.
+constexpr size_t kTailPad = 256;
 static int dalloc(void** p, size_t bytes) {
-  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+  if (hipMalloc(p, bytes + kTailPad) != hipSuccess) {
     *p = nullptr;
     return fail(P3D_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
   }
@@ -998,8 +1002,8 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
       break;
     case P3D_OPT_STENCIL: {
       const int v = (int)value;
-      if (v != -1 && v != 0 && v != 1)
-        return fail(P3D_EINVAL, "P3D_OPT_STENCIL must be -1, 0 or 1");
+      if (v < -1 || v > 4)
+        return fail(P3D_EINVAL, "P3D_OPT_STENCIL must be -1 .. 4");
       if (v != s->L.stencil) {
         // captured iterations hold the other kernel
         HIPCHK(hipStreamSynchronize(s->stream));

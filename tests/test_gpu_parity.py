@@ -65,9 +65,10 @@ KERNEL_GRIDS = [(9, 8, 12), (13, 11, 7), (33, 45, 70), (70, 27, 40), (140, 19, 2
 @pytest.mark.parametrize("dims", KERNEL_GRIDS)
 @pytest.mark.parametrize("pm_r1", [-1.0, 1.0])
 def test_operator_kernels_agree(oracle_mod, solver_mod, dims, pm_r1):
-    """The row-blocked operator kernel (P3D_OPT_STENCIL=1) gives the column
-    march's values bit for bit, in FP64 and in the FP32 operator of the
-    mixed recurrence; the FP64 values against the oracle as test_operator."""
+    """The row-blocked and the LDS-staged operator kernels (P3D_OPT_STENCIL=1,
+    and 3 and 4: tall and short patches) give the column march's values bit
+    for bit, in FP64 and in the FP32 operator of the mixed recurrence; the
+    FP64 values against the oracle as test_operator."""
     from pot3d_amd import _lib
     say(f"[gpu] operator kernels {dims} pm_r1={pm_r1}")
     nr, nt, np_ = dims
@@ -79,13 +80,14 @@ def test_operator_kernels_agree(oracle_mod, solver_mod, dims, pm_r1):
             s.set_option(_lib.OPT_APPLY_FP32, fp32)
             s.set_option(_lib.OPT_STENCIL, 0)
             y0 = s.ax(x)
-            s.set_option(_lib.OPT_STENCIL, 1)
-            y1 = s.ax(x)
-            assert np.array_equal(y0.view(np.uint64), y1.view(np.uint64)), fp32
             ref = o.ax(x)
             scale = np.abs(o.coefs()).sum(axis=0) * np.abs(x).max()
             tol = 1e-14 if not fp32 else 1e-6
-            assert np.max(np.abs(y1 - ref) / scale) < tol
+            for kernel in (1, 3, 4):
+                s.set_option(_lib.OPT_STENCIL, kernel)
+                y1 = s.ax(x)
+                assert np.array_equal(y0.view(np.uint64), y1.view(np.uint64)), (fp32, kernel)
+                assert np.max(np.abs(y1 - ref) / scale) < tol
 
 
 @pytest.mark.parametrize("name", ["tiny", "mid"])
