@@ -124,12 +124,8 @@ struct p3d_solver {
 };
 
 // ---------------------------------------------------------------------------
-// Every device buffer gets kTailPad bytes past its end: the 16-byte vector
-// loads of the LDS-staged operator (k_stencil_lds) read whole aligned vectors,
-// the last of which may reach past the last value.
-constexpr size_t kTailPad = 256;
 static int dalloc(void** p, size_t bytes) {
-  if (hipMalloc(p, bytes + kTailPad) != hipSuccess) {
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
     *p = nullptr;
     return fail(P3D_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
   }
