@@ -1010,7 +1010,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t n1 = n_all;
   const float alpha = (float)st_load(&st->alpha);
   double s1 = 0.0;
-  constexpr int kU = 8;  // points per thread in flight
+  constexpr int kU = 16;  // points per thread in flight (measured: 8 -> 16, 4% faster)
   // grid-stride over runs of kU kBlock points in dispatch order: the
   // workgroups in flight stream one window of the vectors together
   for (uint32_t nb = blockIdx.x * (kU * kBlock) + threadIdx.x; nb < n1;
@@ -1153,7 +1153,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   const float* __restrict__ pin = pb.p[T_];
   float* __restrict__ pout = pb.p[(T_ + 1) % kXBatch];
-  constexpr int kU = 8;
+  constexpr int kU = 8;  // (16 measured 4% slower: half the resident waves)
   // grid-stride over runs of kU kBlock points in dispatch order: the
   // workgroups in flight stream one window of the vectors together
   for (uint32_t nb = blockIdx.x * (kU * kBlock) + threadIdx.x; nb < n1;
