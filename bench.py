@@ -427,7 +427,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(tbest)), 2),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 inner", "data": "synthetic (seeded bipole+harmonic Br)",
-        "config": {"workload": f"cfg{args.config} {label} {gname}, r in [{dom['r0']}, "
+        "config": {"workload": f"{label} {gname}, r in [{dom['r0']}, "
                                f"{dom['r1']}], {dom['n_bipoles']} bipoles, option "
                                f"'{dom['option']}', ILU0 PCG to ||r||2/||b||2<=1e-9",
                    "unknowns": N, "precond": "ILU0 (D-ILU); fast paths block-Jacobi ILU0",

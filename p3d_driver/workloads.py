@@ -137,18 +137,18 @@ def config(idx: int, seed: int = 0) -> Workload:
       2  151x301x601 PFSS, ifprec=2 (ILU0)
       3  51x91x181 'potential' (Neumann dphi/dr=0 at r1), ILU0
       4  151x301x601 PFSS, ILU0, held-out seed for the warm start trained
-         on 51x91x181 (the cfg2 problem of that seed)
+         on 51x91x181 (workload 2 with that seed)
       5  301x601x1201 'open', r in [2.5,10], 16 bipoles, ILU0"""
     if idx == 1:
-        return pfss(51, 91, 181, seed, ifprec=1, name=f"cfg1_51x91x181_s{seed}")
+        return pfss(51, 91, 181, seed, ifprec=1, name=f"pfss_diag_51x91x181_s{seed}")
     if idx == 2:
-        return pfss(151, 301, 601, seed, ifprec=2, name=f"cfg2_151x301x601_s{seed}")
+        return pfss(151, 301, 601, seed, ifprec=2, name=f"pfss_151x301x601_s{seed}")
     if idx == 3:
         return pfss(51, 91, 181, seed, option="potential", ifprec=2,
-                    name=f"cfg3_51x91x181_potential_s{seed}")
+                    name=f"potential_51x91x181_s{seed}")
     if idx == 4:
-        return pfss(151, 301, 601, seed, ifprec=2, name=f"cfg4_151x301x601_s{seed}")
+        return pfss(151, 301, 601, seed, ifprec=2, name=f"pfss_heldout_151x301x601_s{seed}")
     if idx == 5:
         return pfss(301, 601, 1201, seed, n_bipoles=16, option="open", r0=2.5, r1=10.0,
-                    drratio=1.03 ** 49, ifprec=2, name=f"cfg5_301x601x1201_s{seed}")
+                    drratio=1.03 ** 49, ifprec=2, name=f"open_301x601x1201_s{seed}")
     raise ValueError(f"no workload {idx}")

@@ -2,20 +2,20 @@
 ABI, against the CPU oracle (where it finishes in seconds) and against the
 FP64 GPU solve (full sizes, size-independent properties).
 
-  cfg1  51x91x181 PFSS ('ss', phi=0 at r1), FP64 PCG with the diagonal
+  51x91x181 PFSS ('ss', phi=0 at r1), FP64 PCG with the diagonal
         preconditioner (ifprec=1): against the oracle's cgsolve.
-  cfg3  the cfg1 grid with ILU0, FP32 inner PCG inside FP64 defect
+  51x91x181 with ILU0, FP32 inner PCG inside FP64 defect
         correction (both mixed modes), and the Neumann variant ('potential',
         dphi/dr=0 at r1, src/pot3d.F90:1250-1255): against the oracle's FP64
         solve; the Neumann system is singular (constants), so phi is compared
         after removing its mean.
-  cfg2  151x301x601 PFSS, ILU0: the FP64 solve's residual recomputed by the
+  151x301x601 PFSS, ILU0: the FP64 solve's residual recomputed by the
         oracle's operator; mixed precision with the block-Jacobi ILU0 against
         the FP64 solve.
-  cfg5  301x601x1201 'open' (r in [2.5, 10], 16 bipoles): mixed precision
+  301x601x1201 'open' (r in [2.5, 10], 16 bipoles): mixed precision
         with the block-Jacobi ILU0 against the FP64 solve.
 
-Bars (the north-star bar at epscg = 1e-9): potential and B-field (getb,
+Bars (the accuracy bars of the solve at epscg = 1e-9): potential and B-field (getb,
 src/pot3d.F90:6187-6233) within 1e-8 relative to their largest magnitude,
 magnetic energy (:6235-6321) within 1e-7 relative, recomputed FP64
 ||b - A x||2/||b||2 <= 1e-9 where the stop is on it (the mixed solves) and
@@ -63,12 +63,12 @@ def _set(s, **opts):
         s.set_option(getattr(_lib, "OPT_" + k.upper()), v)
 
 
-def test_cfg1_pfss_fp64_diag_vs_oracle(oracle_mod):
-    """cfg1: FP64 PCG, diagonal preconditioner, the reference's epsn stop."""
+def test_pfss_51x91x181_fp64_diag_vs_oracle(oracle_mod):
+    """51x91x181 PFSS: FP64 PCG, diagonal preconditioner, the reference's epsn stop."""
     from pot3d_amd import solver
     from p3d_driver import workloads
     w = workloads.config(1)
-    say(f"[gpu] cfg1 {w.name} N={w.N}")
+    say(f"[gpu] {w.name} N={w.N}")
     o = oracle_mod.Oracle(w.mesh, w.pm_r1)
     b = o.rhs(w.br0)
     ref = o.cgsolve(b, ifprec=1, epscg=1e-9)
@@ -87,14 +87,14 @@ def test_cfg1_pfss_fp64_diag_vs_oracle(oracle_mod):
 
 @pytest.mark.parametrize("option", ["ss", "potential"])
 @pytest.mark.parametrize("mixed", [1, 2])
-def test_cfg3_mixed_ilu_vs_oracle(oracle_mod, option, mixed):
-    """cfg3: FP32 inner ILU0-PCG (inner tolerance 1e-5 in mode 1) inside
-    FP64 defect correction to ||b - A x||2/||b||2 <= 1e-9, on the cfg1 grid,
+def test_51x91x181_mixed_ilu_vs_oracle(oracle_mod, option, mixed):
+    """51x91x181: FP32 inner ILU0-PCG (inner tolerance 1e-5 in mode 1) inside
+    FP64 defect correction to ||b - A x||2/||b||2 <= 1e-9, on the 51x91x181 grid,
     with phi=0 at r1 ('ss') and the Neumann variant ('potential')."""
     from pot3d_amd import solver
     from p3d_driver import workloads
     w = workloads.pfss(51, 91, 181, seed=0, option=option, ifprec=2)
-    say(f"[gpu] cfg3 {w.name} option={option} mixed={mixed}")
+    say(f"[gpu] {w.name} option={option} mixed={mixed}")
     o = oracle_mod.Oracle(w.mesh, w.pm_r1)
     b = o.rhs(w.br0)
     ref = o.cgsolve(b, ifprec=3, epscg=1e-12)
@@ -159,12 +159,12 @@ def _fp64_then_mixed(w, blocks, label, oracle_check=None):
         _check_fields(s, phi, phi64, B64, w64, label=f"{label} mixed vs fp64")
 
 
-def test_cfg2_fp64_residual_and_mixed(oracle_mod):
-    """cfg2 151x301x601: the FP64 ILU0 solve's residual recomputed by the
+def test_151x301x601_fp64_residual_and_mixed(oracle_mod):
+    """151x301x601: the FP64 ILU0 solve's residual recomputed by the
     oracle's operator; mixed block-Jacobi solve against it."""
     from p3d_driver import workloads
     w = workloads.config(2)
-    say(f"[gpu] cfg2 {w.name} N={w.N}")
+    say(f"[gpu] {w.name} N={w.N}")
 
     def oracle_residual(s, x):
         o = oracle_mod.Oracle(w.mesh, w.pm_r1)
@@ -173,23 +173,23 @@ def test_cfg2_fp64_residual_and_mixed(oracle_mod):
         say(f"    oracle ||b - A x||/||b|| = {rr:.3e}")
         assert rr <= 1.05e-9
 
-    _fp64_then_mixed(w, (8, 16, 16), "cfg2", oracle_residual)
+    _fp64_then_mixed(w, (8, 16, 16), "151x301x601", oracle_residual)
 
 
 @pytest.mark.timeout(600)
-def test_cfg5_open_mixed_vs_fp64():
-    """cfg5 301x601x1201 'open': mixed block-Jacobi solve against the FP64
+def test_301x601x1201_open_mixed_vs_fp64():
+    """301x601x1201 'open': mixed block-Jacobi solve against the FP64
     ILU0 solve (phi, B, energy; recomputed residuals of both)."""
     from p3d_driver import workloads
     w = workloads.config(5)
-    say(f"[gpu] cfg5 {w.name} N={w.N}")
-    _fp64_then_mixed(w, (8, 16, 16), "cfg5")
+    say(f"[gpu] {w.name} N={w.N}")
+    _fp64_then_mixed(w, (8, 16, 16), "301x601x1201")
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("cfg,family", [(4, "pfss"), (5, "open")])
-def test_learned_warm_start_real_grids(cfg, family):
-    """cfg4 (151x301x601 PFSS) and cfg5 (301x601x1201 'open'): the family's
+@pytest.mark.parametrize("wl,family", [(4, "pfss"), (5, "open")])
+def test_learned_warm_start_real_grids(wl, family):
+    """151x301x601 PFSS and 301x601x1201 'open': the family's
     network trained on 51x91x181 PCG solutions (committed weights, held-out
     seed 0) predicts on 51x91x181, is prolonged to the configuration's grid
     and starts the value path's solve (mixed precision, block-Jacobi ILU0):
@@ -203,9 +203,9 @@ def test_learned_warm_start_real_grids(cfg, family):
     if not os.path.exists(wpath):
         pytest.skip(f"no trained weights at pot3d_amd/weights/ (python -m "
                     f"p3d_driver.train_warmstart --domain {family} writes them)")
-    w = workloads.config(cfg)
+    w = workloads.config(wl)
     wc = workloads.domain_workload(family, workloads.COARSE_GRID, 0)
-    say(f"[gpu] cfg{cfg} {w.name} N={w.N} warm start from {os.path.basename(wpath)}")
+    say(f"[gpu] {w.name} N={w.N} warm start from {os.path.basename(wpath)}")
     net, meta = W.WarmStartNet.load(wpath, wc.mesh, wc.pm_r1)
     with solver.Pot3dSolver(wc.mesh, wc.pm_r1, 2) as sc:
         sc.set_br0(wc.br0)
@@ -230,4 +230,4 @@ def test_learned_warm_start_real_grids(cfg, family):
         say(f"    zero guess {n0} iterations; learned guess (rel err {e:.3f}) {st['ncg']} "
             f"iterations, guess {st['guess']}, true relres {st['true_relres']:.2e}")
         assert st["guess"] == 1 and st["ncg"] < n0 and st["true_relres"] <= 1e-9
-        _check_fields(s, s.fill_phi(x), phi0, B0, w0, label=f"cfg{cfg} learned vs zero guess")
+        _check_fields(s, s.fill_phi(x), phi0, B0, w0, label=f"{w.name} learned vs zero guess")
