@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged operator march (Launch::stencil = 2; y = A x): the operator and
+// LDS-staged operator march (Launch::stencil = -1, 2, 3, 4; y = A x): the operator and
 // the per-point arithmetic of k_stencil_march, with every value of x loaded
 // from memory once per workgroup.  A workgroup of four waves owns a patch of
 // kLdsW = 64 consecutive i by 4 WJ theta rows and marches a segment of phi
