@@ -873,9 +873,13 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     int slot, Epi epi, double* out, bool guard, hipStream_t s,
                     const TileMap& tm) {
   const int want = (epi != EPI_NONE);
-  // the LDS-staged march (y = A x only) stores through 32-bit buffer offsets
-  // (the output, tile padding included, is below 2 N values)
-  const bool lds_ok = mode == 0 && 2 * (double)g.N * sizeof(T) < (double)kDropStore;
+  // the LDS-staged march (y = A x only) stores through 32-bit buffer
+  // offsets: every output offset, tile padding included, below kDropStore
+  const double nout =
+      tm.on ? (double)tm.TNI * tm.TNJ * 64.0 *
+                  (double)(((g.c + (1 << tm.tk_log2) - 1) >> tm.tk_log2) << tm.tk_log2)
+            : (double)g.N;
+  const bool lds_ok = mode == 0 && nout * sizeof(T) < (double)kDropStore;
   if (lds_ok && (L.stencil < 0 || L.stencil >= 2)) {
     const bool tall = L.stencil == 3 || ((L.stencil < 0 || L.stencil == 2) &&
                                          stencil_lds_tall<T>(g));
