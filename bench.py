@@ -144,19 +144,36 @@ def measure_kernels(s, N: int, padded: int, nr: int, reps: int = 20, graph: bool
 
 # Kernels of the value path's iteration: FP32 recurrence with the
 # block-Jacobi ILU0 on register tiles.  (name, time_kernel id, algorithmic
-# bytes per unknown, padded-array bytes per point).
-VALUE_KERNELS = [
+# bytes per unknown, padded-array bytes per point).  From 2^26 unknowns the
+# library folds the residual update into the ILU's forward sweep
+# (P3D_OPT_FUSE_RESIDUAL = -1): one kernel sequence or the other.
+FUSE_RESIDUAL_MIN_N = 1 << 26
+VALUE_KERNELS_FUSED = [
+    # r, ap, dinv read, r, y written (forward); y, dinv read, z written
+    ("ilu_apply_fp32_blockjacobi_with_r_update (k_tile_fwd_reg<float, 16, false, true>"
+     "+k_tile_bwd_reg<float>)", 11, 32, 0),
+]
+VALUE_KERNELS_SEPARATE = [
     ("ilu_apply_fp32_blockjacobi (k_tile_fwd_reg+k_tile_bwd_reg<float>)", 7, 24, 0),
+    # r, ap read, r written (tile-major)
+    ("r_update_fp32 (k_r_update)", 9, 12, 0),
+]
+VALUE_KERNELS_REST = [
     ("stencil_fp32 (k_stencil_lds<float>)", 1, 4, 4),
     # per iteration of a group of 4: p, z (tile-major, gathered) read, p
     # written (12 B) on every iteration, and on the group's last one also
     # the FP64 solution read and written (16) and the group's 3 earlier
     # directions read (12): (4*12 + 28) / 4 = 19 B
     ("p_update_fp32_fp64x (k_p_update_x<float>, x every 4 iterations)", 8, 19, 0),
-    # r, ap read, r written (tile-major)
-    ("r_update_fp32 (k_r_update)", 9, 12, 0),
-    ("rz_dot_fp32 (k_dot2<float>, tile order)", 10, 8, 0),
+    # r, z read: r.z and r.r
+    ("rz_rr_dot_fp32 (k_dot2<float>, tile order)", 10, 8, 0),
 ]
+
+
+def value_kernels(N: int):
+    """The value path's kernel sequence at N unknowns (its ILU apply first)."""
+    head = VALUE_KERNELS_FUSED if N >= FUSE_RESIDUAL_MIN_N else VALUE_KERNELS_SEPARATE
+    return head + VALUE_KERNELS_REST
 
 
 def set_fast_ilu(s, args):
@@ -177,12 +194,13 @@ def measure_value_kernels(s, N: int, padded: int, nr: int, reps: int = 5):
     measure_kernels; the ILU apply is the dominant one, so it is the
     roofline kernel.  Also the algorithmic bytes of one iteration."""
     table = {}
-    for name, kid, per_n, per_pad in VALUE_KERNELS:
+    kernels = value_kernels(N)
+    for name, kid, per_n, per_pad in kernels:
         ms = s.time_kernel(kid, reps)
         nbytes = per_n * N + per_pad * padded
         table[name] = {"ms": round(ms, 5), "bytes": nbytes,
                        "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
-    name = VALUE_KERNELS[0][0]
+    name = kernels[0][0]
     iter_bytes = sum(v["bytes"] for v in table.values())
     iter_ms = sum(v["ms"] for v in table.values())
     return {"table": table, "dominant": dict(name=name, **table[name]),

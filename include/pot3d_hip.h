@@ -141,9 +141,18 @@ enum {
                                 whose 62-lane waves cover the r extent to
                                 90% or more) or 0.  Same operator values
                                 bit for bit. */
-  P3D_OPT_APPLY_FP32 = 16   /* p3d_apply_A: 1 applies the FP32 operator of
+  P3D_OPT_APPLY_FP32 = 16,  /* p3d_apply_A: 1 applies the FP32 operator of
                                 the mixed-precision recurrence (x rounded to
                                 float, the result in float); 0 FP64 */
+  P3D_OPT_FUSE_RESIDUAL = 17 /* FP32 recurrence with the register-tile ILU
+                                sweeps (P3D_OPT_TILE_STAGE 2): 1 folds the
+                                residual update r -= alpha Ap into the
+                                forward sweep, 0 runs it on its own; -1
+                                (default) 1 when N >= 2^26 unknowns (the
+                                sweeps are then bandwidth-bound and the
+                                fold saves a pass; below, they are
+                                latency-bound and it costs).  Same values
+                                bit for bit. */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /
@@ -249,8 +258,10 @@ int p3d_history(p3d_solver* s, double* hist, int n);
  * 5 FP64 p update, 6 plain copy of N doubles (bandwidth reference), 7 ILU
  * apply in FP32; with the single-wave ILU tile sweeps (tile-major vectors,
  * else P3D_ESTATE) the FP32 recurrence's 8 p update with the FP64 solution
- * update, 9 residual update, 10 r.z.  The work vectors' contents are
- * left undefined (a later solve sets them up).
+ * update, 9 residual update, 10 r.z with r.r, and with the register tiles
+ * (P3D_OPT_TILE_STAGE 2) 11 the FP32 ILU apply with the residual update
+ * folded into its forward sweep (as the recurrence runs it).  The work
+ * vectors' contents are left undefined (a later solve sets them up).
  * Average milliseconds per launch over reps launches. */
 int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms);
 

@@ -1003,17 +1003,17 @@ void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec
 }
 
 // The residual update of the FP32 recurrence with the FP64 solution, one
-// contiguous pass over the tile-major r and ap: r -= alpha ap, r.r (EPI_RR);
-// entries outside the grid are zero on both.  (k_cg_update would read r
-// through the tile map.)  The solution update goes with the p update.
+// contiguous pass over the tile-major r and ap: r -= alpha ap (entries
+// outside the grid are zero on both; r.r goes with r.z after the
+// preconditioner).  The register-tile forward sweep folds the same update
+// into its loads when it can (dilu_tiles_fuse_rupd): the same fmaf, so the
+// same bits.  The solution update goes with the p update.
 __global__ void __launch_bounds__(kBlock)
-    k_r_update(uint32_t n_all, int G, int CH, float* __restrict__ r, const float* __restrict__ ap,
-               CgState* st, double* partial, unsigned* counter) {
+    k_r_update(uint32_t n_all, int G, float* __restrict__ r, const float* __restrict__ ap,
+               const CgState* st) {
   if (st_done(st)) return;
-  const int Lb = logical_block(G);
   const uint32_t n1 = n_all;
   const float alpha = (float)st_load(&st->alpha);
-  double s1 = 0.0;
   constexpr int kU = 16;  // points per thread in flight (measured: 8 -> 16, 4% faster)
   // grid-stride over runs of kU kBlock points in dispatch order: the
   // workgroups in flight stream one window of the vectors together
@@ -1032,21 +1032,14 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < kU; u++) {
       const uint32_t n = nb + u * kBlock;
-      if (n < n1) {
-        const float rn = rv[u] - alpha * av[u];
-        r[n] = rn;
-        s1 += (double)rn * (double)rn;
-      }
+      if (n < n1) r[n] = __builtin_fmaf(-alpha, av[u], rv[u]);
     }
   }
-  double v[2] = {0.0, s1};
-  block_finish<2>(v, partial, counter, G, Lb, st, EPI_RR, nullptr);
 }
 
-void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
-                     Reduce red, int slot, hipStream_t s) {
-  k_r_update<<<Lt.G, kBlock, 0, s>>>(n_all, Lt.G, Lt.CH, r, ap, st, red.partial,
-                                      red.counter + (size_t)slot * kRedCounters);
+void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap,
+                     const CgState* st, hipStream_t s) {
+  k_r_update<<<Lt.G, kBlock, 0, s>>>(n_all, Lt.G, r, ap, st);
 }
 
 // p = beta p + z  (:4361-4363)

@@ -124,9 +124,10 @@ void launch_stencil(const Metric& m, const Grid3& g, const Launch& L, const T* x
                     const TileMap& tm = TileMap{});
 
 // Residual update of the FP32 recurrence: r -= alpha ap over n_all
-// tile-major entries, r.r (EPI_RR); Lt is the launch shape of n_all.
-void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap, CgState* st,
-                     Reduce red, int slot, hipStream_t s);
+// tile-major entries (r.r is reduced with r.z after the preconditioner); Lt
+// is the launch shape of n_all.
+void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap,
+                     const CgState* st, hipStream_t s);
 
 // CG update x += alpha p, r -= alpha ap, and for PREC_DIAG/NONE the reduction
 // of r.z and r.r with epilogue EPI_RITER; for PREC_ILU only r.r (EPI_RR).

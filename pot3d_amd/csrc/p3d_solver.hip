@@ -113,6 +113,7 @@ struct p3d_solver {
   int guess_check = 0;
   int tile_k = 32, tile_waves = 1, tile_stage = 0;
   int apply_fp32 = 0;  // p3d_apply_A with the FP32 operator
+  int fuse_rupd = -1;  // P3D_OPT_FUSE_RESIDUAL
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
   // graphs: [0] FP64, [1] FP32, [2] FP32 recurrence with the FP64 solution
@@ -222,12 +223,23 @@ static int read_state(p3d_solver* s) {
 
 // ---------------------------------------------------------------------------
 // Preconditioner application r -> z (+ fused dots with epilogue).
+// ap_upd (FP32 register tiles, guard set): the residual update r -= alpha ap
+// is folded into the forward sweep (r is written).
 template <typename T>
-static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, bool guard) {
+static void prec_and_dot(p3d_solver* s, const T* r, T* z, Epi epi, double* out, bool guard,
+                         const float* ap_upd = nullptr) {
   if (s->ifprec == PREC_ILU) {
     const CgState* gd = guard ? s->st : nullptr;
     if (s->sweep_mode == 0 && s->sweep_ready) {
-      launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
+      if constexpr (sizeof(T) == 4) {
+        if (ap_upd)
+          launch_dilu_tiles_rupd(s->sweep, s->m, s->g, const_cast<float*>(r), ap_upd, z, gd,
+                                 s->stream);
+        else
+          launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
+      } else {
+        launch_dilu_tiles<T>(s->sweep, s->m, s->g, s->dinv, r, s->yv, z, gd, s->stream);
+      }
     } else {
       launch_dilu_solve_levels<T>(s->m, s->g, s->dinv, r, s->yv, z, nullptr, true, s->blocks, gd,
                                   s->stream);
@@ -259,8 +271,12 @@ static void cg_iteration(p3d_solver* s, Vecs<T> v, int t) {
       launch_pole_sums<float>(s->m, s->g, p, s->pole_part, s->stream);
       launch_stencil<float>(s->m, s->g, s->L, p, v.ap, nullptr, 0, s->pole_part, s->st, s->red,
                             0, EPI_PAP, nullptr, true, s->stream, s->tm);
-      launch_r_update(s->gt.N, s->Lt, v.r, v.ap, s->st, s->red, 2, s->stream);
-      prec_and_dot<float>(s, v.r, v.z, EPI_RZITER, nullptr, true);
+      // r -= alpha ap, in the forward sweep of the register tiles where they
+      // run, else on its own; then z = M^-1 r and r.z, r.r together
+      const bool fused = (s->fuse_rupd == 1 || (s->fuse_rupd < 0 && s->g.N >= (1u << 26))) &&
+                         s->sweep_mode == 0 && s->sweep_ready && dilu_tiles_fuse_rupd(s->sweep);
+      if (!fused) launch_r_update(s->gt.N, s->Lt, v.r, v.ap, s->st, s->stream);
+      prec_and_dot<float>(s, v.r, v.z, EPI_RITER, nullptr, true, fused ? v.ap : nullptr);
       launch_p_update_x(s->g, s->L, t, v.pb, v.z, s->st, s->stream, s->tm, v.xs, v.xscale,
                         s->red.counter + 5 * kRedCounters);
       return;
@@ -1014,6 +1030,20 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
     case P3D_OPT_APPLY_FP32:
       s->apply_fp32 = value != 0;
       break;
+    case P3D_OPT_FUSE_RESIDUAL: {
+      const int v = (int)value;
+      if (v < -1 || v > 1) return fail(P3D_EINVAL, "P3D_OPT_FUSE_RESIDUAL must be -1, 0 or 1");
+      if (v != s->fuse_rupd) {
+        // captured iterations hold the other kernel sequence
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < 3; q++) {
+          if (s->gexec[q]) hipGraphExecDestroy(s->gexec[q]);
+          s->gexec[q] = nullptr;
+        }
+        s->fuse_rupd = v;
+      }
+      break;
+    }
     case P3D_OPT_TILE_K:
     case P3D_OPT_TILE_WAVES:
     case P3D_OPT_TILE_STAGE: {
@@ -1282,9 +1312,12 @@ int p3d_ilu_dinv(p3d_solver* s, float* dinv) {
 int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   if (!s || !avg_ms || reps <= 0) return fail(P3D_EINVAL, "bad argument");
   if (which == 2 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
-  if (which == 7 && s->ifprec != 2) return fail(P3D_ESTATE, "no ILU preconditioner");
-  if ((which == 8 || which == 9 || which == 10) && !s->tm.on)
+  if ((which == 7 || which == 11) && s->ifprec != 2)
+    return fail(P3D_ESTATE, "no ILU preconditioner");
+  if ((which == 8 || which == 9 || which == 10 || which == 11) && !s->tm.on)
     return fail(P3D_ESTATE, "no tile-major placement (single-wave ILU tile sweeps)");
+  if (which == 11 && !(s->sweep_mode == 0 && s->sweep_ready && dilu_tiles_fuse_rupd(s->sweep)))
+    return fail(P3D_ESTATE, "no register-tile ILU sweeps (P3D_OPT_TILE_STAGE 2)");
   if (which == 1 || which == 3 || which >= 7) {
     int rc = ensure_mixed(s);
     if (!rc && which == 8) rc = ensure_pbuf(s);
@@ -1374,11 +1407,14 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
         break;
       }
       case 9:  // residual update of the FP32 recurrence (tile-major r, ap)
-        launch_r_update(s->gt.N, s->Lt, s->r32, s->ap32, s->st, s->red, 2, s->stream);
+        launch_r_update(s->gt.N, s->Lt, s->r32, s->ap32, s->st, s->stream);
         break;
-      case 10:  // r.z of the FP32 recurrence, contiguous in tile order
-        launch_dot2<float>(s->gt, s->Lt, s->r32, s->z32, nullptr, nullptr, s->st, s->red, 1,
+      case 10:  // r.z and r.r of the FP32 recurrence, contiguous in tile order
+        launch_dot2<float>(s->gt, s->Lt, s->r32, s->z32, s->r32, s->r32, s->st, s->red, 1,
                            EPI_NONE, nullptr, false, s->stream);
+        break;
+      case 11:  // FP32 ILU apply with the residual update in its forward sweep
+        launch_dilu_tiles_rupd(s->sweep, s->m, s->g, s->r32, s->ap32, s->z32, s->st, s->stream);
         break;
       default:
         break;
@@ -1395,7 +1431,7 @@ int p3d_time_kernel(p3d_solver* s, int which, int reps, double* avg_ms) {
   hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   *avg_ms = (double)ms / reps / (which == 8 ? kXBatch : 1);
-  if (which > 10 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
+  if (which > 11 || which < 0) return fail(P3D_EINVAL, "unknown kernel id %d", which);
   return P3D_OK;
 }
 

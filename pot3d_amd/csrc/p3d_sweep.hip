@@ -846,10 +846,14 @@ __device__ __forceinline__ void reg_stage(TileRegLds<TK>& sh, const RegIn<TK>& i
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-template <typename T, int TK, bool kWrap>
+// kUpd: the residual update of the FP32 recurrence on the way in,
+// r = r - alpha Ap (the tile's r is stored back by the caller).
+template <typename T, int TK, bool kWrap, bool kUpd = false>
 __device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, const Grid3& g,
                                              const DiagArgs& da, const float* coup,
-                                             const T* __restrict__ r, int code, int lane) {
+                                             const T* __restrict__ r, int code, int lane,
+                                             const float* __restrict__ ap = nullptr,
+                                             float alpha = 0.f) {
   const RegTile<TK> t(da, g, code, lane);
   const size_t fbi = tile_id(da, max(t.I - 1, 0), t.J, t.Kc) * TK * 8;
   const size_t fbj = tile_id(da, t.I, max(t.J - 1, 0), t.Kc) * TK * 8;
@@ -869,7 +873,10 @@ __device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, con
 #pragma unroll
   for (int u = 0; u < TK; u++) {
     in.d[u] = da.dinv_t[t.tb + (size_t)u * 64];
-    in.v[u] = (float)r[t.tb + (size_t)u * 64];
+    if constexpr (kUpd)
+      in.v[u] = __builtin_fmaf(-alpha, ap[t.tb + (size_t)u * 64], (float)r[t.tb + (size_t)u * 64]);
+    else
+      in.v[u] = (float)r[t.tb + (size_t)u * 64];
   }
 }
 
@@ -1077,10 +1084,12 @@ __device__ __forceinline__ void bwd_reg_tile(RegIn<TK>& in, TileRegLds<TK>& sh, 
 
 // Waves per SIMD the register tiles are built for: 7 forward and 8
 // backward (at most 72 and 64 VGPRs) for the FP32 16-plane tiles without
-// the phi seam, which then fit without spilling; the others take what they
-// need.
-template <typename T, int TK, bool kWrap, bool kFwd>
-constexpr int kRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? (kFwd ? 7 : 8) : 1;
+// the phi seam, which then fit without spilling (6 for the forward sweep
+// that also updates the residual: its Ap values are in flight with r); the
+// others take what they need.
+template <typename T, int TK, bool kWrap, bool kFwd, bool kUpd = false>
+constexpr int kRegWaves =
+    (TK == 16 && !kWrap && sizeof(T) == 4) ? (kFwd ? (kUpd ? 6 : 7) : 8) : 1;
 // Waves per workgroup: each marches a tile of its own (no data shared
 // between the waves), two per workgroup so that the resident tiles per CU
 // are not capped by the number of workgroups a CU holds.  (Looping a wave
@@ -1088,11 +1097,15 @@ constexpr int kRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? (kFwd ? 7 : 8
 // the second operand set halves the resident waves.)
 constexpr int kRegTilesPerGroup = 2;
 
-template <typename T, int TK, bool kWrap>
+// kUpd (FP32, with guard): the residual update r -= alpha Ap folded into
+// the forward sweep, each tile storing its updated r before its march; the
+// same fmaf as k_r_update, so r and the sweep's values are the same bits.
+template <typename T, int TK, bool kWrap, bool kUpd = false>
 __global__ void __launch_bounds__(64 * kRegTilesPerGroup)
-    __attribute__((amdgpu_waves_per_eu(kRegWaves<T, TK, kWrap, true>, 8)))
+    __attribute__((amdgpu_waves_per_eu(kRegWaves<T, TK, kWrap, true, kUpd>, 8)))
     k_tile_fwd_reg(Metric m, Grid3 g, DiagArgs da, const float* __restrict__ coup,
-                   const T* __restrict__ r, const CgState* guard) {
+                   T* __restrict__ r, const CgState* guard,
+                   const float* __restrict__ ap = nullptr) {
   static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
   __shared__ TileRegLds<TK> shw[kRegTilesPerGroup];
   const int lane = threadIdx.x & 63;
@@ -1101,9 +1114,16 @@ __global__ void __launch_bounds__(64 * kRegTilesPerGroup)
   if (b >= da.count) return;  // no workgroup barrier follows
   const int code = reg_tile_code(da, b);
   RegIn<TK> in;
-  fwd_reg_load<T, TK, kWrap>(in, m, g, da, coup, r, code, lane);
+  float alpha = 0.f;
+  if constexpr (kUpd) alpha = (float)st_load(&guard->alpha);
+  fwd_reg_load<T, TK, kWrap, kUpd>(in, m, g, da, coup, r, code, lane, ap, alpha);
   // the done flag is read after the tile's loads, before any store
   if (guard && st_done(guard)) return;
+  if constexpr (kUpd) {
+    const RegTile<TK> t(da, g, code, lane);
+#pragma unroll
+    for (int u = 0; u < TK; u++) r[t.tb + (size_t)u * 64] = in.v[u];
+  }
   fwd_reg_tile<TK, kWrap>(in, shw[wave], m, g, da, code, lane);
 }
 
@@ -1146,19 +1166,33 @@ __global__ void k_reg_coup(Metric m, Grid3 g, int TNI, int TNJ, float* __restric
   u[128] = cu.i;
 }
 
+// ap non-null (FP32, with guard): the forward sweep also applies the
+// residual update r -= alpha Ap (k_tile_fwd_reg<.., kUpd>).
 template <typename T, int TK>
 void tiles_launch_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, const T* r, T* z,
-                      const CgState* guard, hipStream_t s) {
+                      const CgState* guard, hipStream_t s, const float* ap = nullptr) {
   const bool wrap = plan.bl.wrap != 0;
   constexpr int kThreads = 64 * kRegTilesPerGroup;
+  T* rw = const_cast<T*>(r);  // written only by the kUpd sweep
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     const DiagArgs da = diag_args(plan, L);
     const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
+    if constexpr (sizeof(T) == 4) {
+      if (ap) {
+        if (wrap)
+          k_tile_fwd_reg<T, TK, true, true>
+              <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard, ap);
+        else
+          k_tile_fwd_reg<T, TK, false, true>
+              <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard, ap);
+        continue;
+      }
+    }
     if (wrap)
-      k_tile_fwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+      k_tile_fwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard);
     else
-      k_tile_fwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+      k_tile_fwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard);
   }
   for (int L = plan.ndiag - 1; L >= 0; L--) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
@@ -1638,6 +1672,19 @@ template <typename T>
 void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
                        const T* r, float* yv, T* z, const CgState* guard, hipStream_t s) {
   tiles_launch_lds<T>(plan, m, g, dinv, r, yv, z, guard, s);
+}
+
+bool dilu_tiles_fuse_rupd(const SweepPlan& plan) {
+  const int w = plan.SI * plan.SJ;
+  return plan.stage_one == 2 && w != 4 && w != 8 && (plan.TK == 16 || plan.TK == 32);
+}
+
+void launch_dilu_tiles_rupd(const SweepPlan& plan, const Metric& m, const Grid3& g, float* r,
+                            const float* ap, float* z, const CgState* guard, hipStream_t s) {
+  if (plan.TK == 16)
+    tiles_launch_reg<float, 16>(plan, m, g, r, z, guard, s, ap);
+  else
+    tiles_launch_reg<float, 32>(plan, m, g, r, z, guard, s, ap);
 }
 
 void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,

@@ -72,6 +72,13 @@ void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, c
                        const T* r, float* yv, T* z, const CgState* guard, hipStream_t s);
 
 // Tile-diagonal sweeps with the stored FP32 couplings of a legacy CSR input.
+// The FP32 register-tile sweeps with the residual update of the recurrence
+// folded into the forward sweep: r -= alpha Ap (alpha from guard, which must
+// be non-null), then z = M^-1 r; available when dilu_tiles_fuse_rupd(plan).
+bool dilu_tiles_fuse_rupd(const SweepPlan& plan);
+void launch_dilu_tiles_rupd(const SweepPlan& plan, const Metric& m, const Grid3& g, float* r,
+                            const float* ap, float* z, const CgState* guard, hipStream_t s);
+
 void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,
                               const float* dinv, const double* r, float* yv, double* z,
                               hipStream_t s);

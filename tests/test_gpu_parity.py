@@ -412,6 +412,17 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
         say(f"  register tiles: ncg={st2['ncg']} replacements={st2['n_outer']}")
         assert st2["ncg"] == st["ncg"] and st2["n_outer"] == st["n_outer"]
         assert np.array_equal(x2, x)
+        # the residual update folded into the register tiles' forward sweep
+        # (the default from 2^26 unknowns): the same fmaf, the same iterates
+        s.set_option(_lib.OPT_FUSE_RESIDUAL, 1)
+        s.time_kernel(11, 1)
+        x3 = np.zeros(s.N)
+        assert s.solve(x3, None, 1e-9, ncgmax=5000)[0] == 0
+        st3 = s.stats()
+        say(f"  residual update in the forward sweep: ncg={st3['ncg']}")
+        assert st3["ncg"] == st["ncg"] and st3["n_outer"] == st["n_outer"]
+        assert np.array_equal(x3, x)
+        s.set_option(_lib.OPT_FUSE_RESIDUAL, -1)
 
 
 def test_failure_paths(oracle_mod, solver_mod):
