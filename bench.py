@@ -154,7 +154,9 @@ VALUE_KERNELS_FUSED = [
      "+k_tile_bwd_reg<float>)", 11, 32, 0),
 ]
 VALUE_KERNELS_SEPARATE = [
-    ("ilu_apply_fp32_blockjacobi (k_tile_fwd_reg+k_tile_bwd_reg<float>)", 7, 24, 0),
+    # below 2^26 unknowns a workgroup per block marches the block's diagonals
+    ("ilu_apply_fp32_blockjacobi (k_tile_fwd_blk+k_tile_bwd_blk<float>, block-persistent)",
+     7, 24, 0),
     # r, ap read, r written (tile-major)
     ("r_update_fp32 (k_r_update)", 9, 12, 0),
 ]

@@ -144,7 +144,7 @@ enum {
   P3D_OPT_APPLY_FP32 = 16,  /* p3d_apply_A: 1 applies the FP32 operator of
                                 the mixed-precision recurrence (x rounded to
                                 float, the result in float); 0 FP64 */
-  P3D_OPT_FUSE_RESIDUAL = 17 /* FP32 recurrence with the register-tile ILU
+  P3D_OPT_FUSE_RESIDUAL = 17, /* FP32 recurrence with the register-tile ILU
                                 sweeps (P3D_OPT_TILE_STAGE 2): 1 folds the
                                 residual update r -= alpha Ap into the
                                 forward sweep, 0 runs it on its own; -1
@@ -153,6 +153,12 @@ enum {
                                 fold saves a pass; below, they are
                                 latency-bound and it costs).  Same values
                                 bit for bit. */
+  P3D_OPT_BLOCK_SWEEP = 18   /* block-Jacobi ILU on register tiles
+                                (P3D_OPT_TILE_STAGE 2 with a cut): 1 runs
+                                each sweep as one launch, a workgroup per
+                                block marching the block's tile diagonals;
+                                0 one launch per diagonal over all blocks;
+                                -1 (default) 1.  Same values bit for bit. */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

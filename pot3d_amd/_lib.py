@@ -65,7 +65,8 @@ SIGNATURES = {
 P3D_OK, P3D_EINVAL, P3D_ENOMEM, P3D_EHIP, P3D_ENOCONV, P3D_ENONFINITE, P3D_ESTATE = range(7)
 OPT_STOP, OPT_MIXED, OPT_INNER_TOL, OPT_CHUNK, OPT_GRAPH, OPT_SWEEP, OPT_VERBOSE, \
     OPT_GUESS_CHECK, OPT_TILE_K, OPT_TILE_WAVES, OPT_ILU_BLOCKS_R, OPT_ILU_BLOCKS_T, \
-    OPT_ILU_BLOCKS_P, OPT_TILE_STAGE, OPT_STENCIL, OPT_APPLY_FP32, OPT_FUSE_RESIDUAL = range(1, 18)
+    OPT_ILU_BLOCKS_P, OPT_TILE_STAGE, OPT_STENCIL, OPT_APPLY_FP32, OPT_FUSE_RESIDUAL, \
+    OPT_BLOCK_SWEEP = range(1, 19)
 
 
 class Stats(C.Structure):

@@ -114,6 +114,7 @@ struct p3d_solver {
   int tile_k = 32, tile_waves = 1, tile_stage = 0;
   int apply_fp32 = 0;  // p3d_apply_A with the FP32 operator
   int fuse_rupd = -1;  // P3D_OPT_FUSE_RESIDUAL
+  int blk_sweep = -1;  // P3D_OPT_BLOCK_SWEEP
   IluBlocks blocks{0, 0, 0, 1};  // block-Jacobi cut of the ILU (none by default)
   int nblk[3] = {1, 1, 1};       // blocks per axis (r, theta, phi)
   // graphs: [0] FP64, [1] FP32, [2] FP32 recurrence with the FP64 solution
@@ -729,6 +730,8 @@ int p3d_destroy(p3d_solver* s) {
   return P3D_OK;
 }
 
+static int block_sweep_on(const p3d_solver* s);
+
 int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const double* dr,
                const double* rh, const double* drh, const double* st, const double* dt,
                const double* sth, const double* dth, const double* dp, const double* dph,
@@ -923,6 +926,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
     if ((rc = factor_ilu(s))) return bail(rc);
     s->sweep_ready = (build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves,
                                        &s->blocks, s->dinv, s->tile_stage, &s->m) == 0);
+    s->sweep.persist = block_sweep_on(s);
     if ((rc = update_tilemap(s))) return bail(rc);
     s->stats.t_setup =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -930,6 +934,14 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(P3D_EHIP, "set-up failed"));
   *out = s;
   return P3D_OK;
+}
+
+// Block-persistent register-tile sweeps (P3D_OPT_BLOCK_SWEEP; -1: below
+// 2^26 unknowns, where one launch per diagonal leaves the device idle at
+// the launch boundaries; above, the blocks' uneven share of the CUs costs
+// more than the launches).
+static int block_sweep_on(const p3d_solver* s) {
+  return s->blk_sweep == 1 || (s->blk_sweep < 0 && s->g.N < (1u << 26));
 }
 
 // Block extents of the block-Jacobi cut for nblk blocks per axis, rounded
@@ -970,6 +982,7 @@ static int rebuild_ilu(p3d_solver* s, bool refactor) {
       build_sweep_plan(s->sweep, s->g, ncu, s->tile_k, s->tile_waves, &s->blocks, s->dinv,
                        s->tile_stage, &s->m);
   s->sweep_ready = prc == 0;
+  s->sweep.persist = block_sweep_on(s);
   int rc = update_tilemap(s);
   if (prc == 2) return fail(P3D_EINVAL, "ILU blocks not aligned with the tiles");
   if (prc == 3)
@@ -1041,6 +1054,21 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
           s->gexec[q] = nullptr;
         }
         s->fuse_rupd = v;
+      }
+      break;
+    }
+    case P3D_OPT_BLOCK_SWEEP: {
+      const int v = (int)value;
+      if (v < -1 || v > 1) return fail(P3D_EINVAL, "P3D_OPT_BLOCK_SWEEP must be -1, 0 or 1");
+      if (v != s->blk_sweep) {
+        // captured iterations hold the other launch sequence
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < 3; q++) {
+          if (s->gexec[q]) hipGraphExecDestroy(s->gexec[q]);
+          s->gexec[q] = nullptr;
+        }
+        s->blk_sweep = v;
+        s->sweep.persist = block_sweep_on(s);
       }
       break;
     }

@@ -1145,6 +1145,86 @@ __global__ void __launch_bounds__(64 * kRegTilesPerGroup)
   bwd_reg_tile<T, TK, kWrap>(in, shw[wave], m, g, da, z, code, lane);
 }
 
+// Block-persistent register-tile sweeps (block-Jacobi plans): one
+// workgroup of kBlkWaves waves per block of the cut marches that block's
+// tile diagonals in order (backward: in reverse), its waves taking the
+// diagonal's tiles in turn, with a workgroup barrier between diagonals.
+// The blocks are independent, so no workgroup ever waits for another: a
+// sweep is one launch instead of one per diagonal, and a block moves on to
+// its next diagonal as soon as its own tiles are done.  A tile's faces come
+// from tiles of the same block on the previous diagonal, written by waves
+// of the same workgroup before the barrier (its release and acquire order
+// them for the workgroup); the per-tile work is that of k_tile_fwd_reg /
+// k_tile_bwd_reg, so the results are the same bits.
+constexpr int kBlkWaves = 8;
+// Waves per SIMD of the block-persistent sweeps: three workgroups per CU
+// for the FP32 16-plane tiles without the seam (the loop over diagonals
+// keeps more values live than one tile does).
+template <typename T, int TK, bool kWrap>
+constexpr int kBlkRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? 6 : 1;
+
+struct BlkArgs {
+  const int* tiles;  // tile codes, by block, then diagonal
+  const int* off;    // nblk * ndiag + 1 offsets into tiles
+  int ndiag;
+};
+
+template <typename T, int TK, bool kWrap, bool kUpd = false>
+__global__ void __launch_bounds__(64 * kBlkWaves)
+    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap>, 8)))
+    k_tile_fwd_blk(Metric m, Grid3 g, DiagArgs da, BlkArgs ba, const float* __restrict__ coup,
+                   T* __restrict__ r, const CgState* guard,
+                   const float* __restrict__ ap = nullptr) {
+  static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
+  __shared__ TileRegLds<TK> shw[kBlkWaves];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // the flag is written by none of this launch's workgroups: every thread
+  // reads the same value and the whole workgroup returns or none does
+  if (guard && st_done(guard)) return;
+  float alpha = 0.f;
+  if constexpr (kUpd) alpha = (float)st_load(&guard->alpha);
+  const int* off = ba.off + (size_t)blockIdx.x * ba.ndiag;
+  for (int L = 0; L < ba.ndiag; L++) {
+    const int o1 = off[L + 1];
+    for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
+      const int code = ba.tiles[b];
+      RegIn<TK> in;
+      fwd_reg_load<T, TK, kWrap, kUpd>(in, m, g, da, coup, r, code, lane, ap, alpha);
+      if constexpr (kUpd) {
+        const RegTile<TK> t(da, g, code, lane);
+#pragma unroll
+        for (int u = 0; u < TK; u++) r[t.tb + (size_t)u * 64] = in.v[u];
+      }
+      fwd_reg_tile<TK, kWrap>(in, shw[wave], m, g, da, code, lane);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int TK, bool kWrap>
+__global__ void __launch_bounds__(64 * kBlkWaves)
+    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap>, 8)))
+    k_tile_bwd_blk(Metric m, Grid3 g, DiagArgs da, BlkArgs ba, const float* __restrict__ coup,
+                   T* __restrict__ z, const CgState* guard) {
+  static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
+  __shared__ TileRegLds<TK> shw[kBlkWaves];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (guard && st_done(guard)) return;  // as in the forward sweep
+  const int* off = ba.off + (size_t)blockIdx.x * ba.ndiag;
+  for (int L = ba.ndiag - 1; L >= 0; L--) {
+    const int o1 = off[L + 1];
+    for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
+      const int code = ba.tiles[b];
+      RegIn<TK> in;
+      bwd_reg_load<T, TK, kWrap>(in, m, g, da, coup, code, lane);
+      bwd_reg_tile<T, TK, kWrap>(in, shw[wave], m, g, da, z, code, lane);
+    }
+    __syncthreads();
+  }
+}
+
 // Couplings of every column of tiles for the register tiles: lower (k, j,
 // i) at coup_lo[(col * 3 + c) * 64 + lane], upper likewise, col = I + TNI J;
 // columns outside the grid take those of column (0, 0), as in the marches.
@@ -1174,6 +1254,37 @@ void tiles_launch_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, co
   const bool wrap = plan.bl.wrap != 0;
   constexpr int kThreads = 64 * kRegTilesPerGroup;
   T* rw = const_cast<T*>(r);  // written only by the kUpd sweep
+  if (plan.persist && plan.blk_tiles) {
+    DiagArgs da = diag_args(plan, 0);
+    da.tiles = nullptr;
+    da.count = 0;
+    const BlkArgs ba{plan.blk_tiles, plan.blk_off, plan.ndiag};
+    const int nb = plan.nblk;
+    constexpr int kT = 64 * kBlkWaves;
+    bool upd = false;
+    if constexpr (sizeof(T) == 4) {
+      if (ap) {
+        upd = true;
+        if (wrap)
+          k_tile_fwd_blk<T, TK, true, true>
+              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard, ap);
+        else
+          k_tile_fwd_blk<T, TK, false, true>
+              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard, ap);
+      }
+    }
+    if (!upd) {
+      if (wrap)
+        k_tile_fwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard);
+      else
+        k_tile_fwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard);
+    }
+    if (wrap)
+      k_tile_bwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+    else
+      k_tile_bwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+    return;
+  }
   for (int L = 0; L < plan.ndiag; L++) {
     const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
     const DiagArgs da = diag_args(plan, L);
@@ -1617,6 +1728,33 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     return 1;
   }
   hipMemcpy(plan.tiles, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (plan.blocked && plan.stage_one == 2) {
+    // the block-persistent register-tile sweeps: every block's tiles by
+    // diagonal, each diagonal in the order of `tiles` (a counting sort on
+    // block-major, diagonal-minor keys)
+    const int nbI = (plan.TNI + bIt - 1) / bIt, nbJ = (plan.TNJ + bJt - 1) / bJt;
+    const int nbK = (plan.NK + bKt - 1) / bKt;
+    plan.nblk = nbI * nbJ * nbK;
+    auto key = [&](int t, int L) {
+      const int I = t % plan.TNI, J = (t / plan.TNI) % plan.TNJ, Kc = t / (plan.TNI * plan.TNJ);
+      return ((size_t)(I / bIt) + (size_t)nbI * ((size_t)(J / bJt) + (size_t)nbJ * (Kc / bKt))) *
+                 plan.ndiag + L;
+    };
+    std::vector<int> bo((size_t)plan.nblk * plan.ndiag + 1, 0);
+    for (int L = 0; L < plan.ndiag; L++)
+      for (int t : by_diag[L]) bo[key(t, L) + 1]++;
+    for (size_t q = 1; q < bo.size(); q++) bo[q] += bo[q - 1];
+    std::vector<int> at(bo.begin(), bo.end() - 1), bt(tl.size());
+    for (int L = 0; L < plan.ndiag; L++)
+      for (int t : by_diag[L]) bt[at[key(t, L)]++] = t;
+    if (hipMalloc((void**)&plan.blk_tiles, bt.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&plan.blk_off, bo.size() * sizeof(int)) != hipSuccess) {
+      free_sweep_plan(plan);
+      return 1;
+    }
+    hipMemcpy(plan.blk_tiles, bt.data(), bt.size() * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(plan.blk_off, bo.data(), bo.size() * sizeof(int), hipMemcpyHostToDevice);
+  }
   if (plan.SI == 1 && plan.SJ == 1) {
     // tile-major dinv and y of the single-wave sweeps: every tile reads and
     // writes whole 256-byte planes instead of 32-byte row pieces
@@ -1665,6 +1803,8 @@ void free_sweep_plan(SweepPlan& plan) {
   if (plan.face_kt) hipFree(plan.face_kt);
   if (plan.coup_lo) hipFree(plan.coup_lo);
   if (plan.coup_up) hipFree(plan.coup_up);
+  if (plan.blk_tiles) hipFree(plan.blk_tiles);
+  if (plan.blk_off) hipFree(plan.blk_off);
   plan = SweepPlan();
 }
 

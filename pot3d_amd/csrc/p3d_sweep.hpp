@@ -49,6 +49,14 @@ struct SweepPlan {
   // column of tiles
   float* coup_lo = nullptr;
   float* coup_up = nullptr;
+  // block-Jacobi register tiles: the tiles of each of the nblk blocks by
+  // diagonal (blk_off: nblk * ndiag + 1 offsets into blk_tiles); persist 1
+  // runs each sweep as one launch, a workgroup per block marching its
+  // diagonals, 0 as one launch per diagonal (the same bits)
+  int nblk = 0;
+  int* blk_tiles = nullptr;
+  int* blk_off = nullptr;
+  int persist = 0;
 };
 
 // Returns 0 when the tile sweeps can run on this grid.
@@ -71,7 +79,6 @@ template <typename T>
 void launch_dilu_tiles(const SweepPlan& plan, const Metric& m, const Grid3& g, const float* dinv,
                        const T* r, float* yv, T* z, const CgState* guard, hipStream_t s);
 
-// Tile-diagonal sweeps with the stored FP32 couplings of a legacy CSR input.
 // The FP32 register-tile sweeps with the residual update of the recurrence
 // folded into the forward sweep: r -= alpha Ap (alpha from guard, which must
 // be non-null), then z = M^-1 r; available when dilu_tiles_fuse_rupd(plan).
@@ -79,6 +86,7 @@ bool dilu_tiles_fuse_rupd(const SweepPlan& plan);
 void launch_dilu_tiles_rupd(const SweepPlan& plan, const Metric& m, const Grid3& g, float* r,
                             const float* ap, float* z, const CgState* guard, hipStream_t s);
 
+// Tile-diagonal sweeps with the stored FP32 couplings of a legacy CSR input.
 void launch_dilu_tiles_stored(const SweepPlan& plan, const CoefArrays& ca, const Grid3& g,
                               const float* dinv, const double* r, float* yv, double* z,
                               hipStream_t s);

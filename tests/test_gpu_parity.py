@@ -422,7 +422,18 @@ def test_mixed_reliable_updates_open_boundary(solver_mod):
         say(f"  residual update in the forward sweep: ncg={st3['ncg']}")
         assert st3["ncg"] == st["ncg"] and st3["n_outer"] == st["n_outer"]
         assert np.array_equal(x3, x)
+        # the register tiles above ran block-persistent (a workgroup per
+        # block marching its diagonals, the default); one launch per
+        # diagonal over all blocks gives the same iterates
+        s.set_option(_lib.OPT_BLOCK_SWEEP, 0)
+        x4 = np.zeros(s.N)
+        assert s.solve(x4, None, 1e-9, ncgmax=5000)[0] == 0
+        st4 = s.stats()
+        say(f"  one launch per tile diagonal: ncg={st4['ncg']}")
+        assert st4["ncg"] == st["ncg"] and st4["n_outer"] == st["n_outer"]
+        assert np.array_equal(x4, x)
         s.set_option(_lib.OPT_FUSE_RESIDUAL, -1)
+        s.set_option(_lib.OPT_BLOCK_SWEEP, -1)
 
 
 def test_failure_paths(oracle_mod, solver_mod):
@@ -627,6 +638,13 @@ def test_block_jacobi_ilu(oracle_mod, solver_mod, nblk):
             s.set_option(_lib.OPT_TILE_STAGE, stage)
             z = s.prec_inv(x)
             assert np.array_equal(z, z_lv), (waves, tk, stage, np.max(np.abs(z - z_lv)))
+            if stage == 2:
+                # register tiles, one launch per diagonal instead of a
+                # workgroup per block marching its diagonals
+                s.set_option(_lib.OPT_BLOCK_SWEEP, 0)
+                z = s.prec_inv(x)
+                s.set_option(_lib.OPT_BLOCK_SWEEP, -1)
+                assert np.array_equal(z, z_lv), ("per diagonal", tk, np.max(np.abs(z - z_lv)))
         zref = o.dilu_apply(x.astype(np.float32), blocks=ext).astype(np.float64)
         assert np.max(np.abs(z_lv - zref)) <= 2e-5 * np.max(np.abs(zref))
         s.set_option(_lib.OPT_TILE_WAVES, 1)
