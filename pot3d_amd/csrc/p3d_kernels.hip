@@ -165,7 +165,22 @@ __global__ void __launch_bounds__(64) k_pole_sums(Metric m, Grid3 g, const T* __
   const int kh = min(g.c, kl + kper);
   const size_t base = (size_t)i + (pole ? (size_t)(g.b - 1) * g.a : 0);
   double acc = 0.0;
-  for (int k = kl; k < kh; k++) acc += (double)x[base + (size_t)g.ab * k] * m.PW[k];
+  // the planes' loads in batches of 16, issued before the batch's sums
+  // (which stay in plane order): one memory round trip per batch
+  constexpr int kB = 16;
+  for (int k = kl; k < kh; k += kB) {
+    T v[kB];
+    double w[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      const int kk = min(k + u, kh - 1);
+      v[u] = x[base + (size_t)g.ab * kk];
+      w[u] = m.PW[kk];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; u++)
+      if (k + u < kh) acc += (double)v[u] * w[u];
+  }
   part[(pole * kPoleSegs + seg) * g.a + i] = acc;
 }
 
