@@ -157,8 +157,12 @@ enum {
                                 (P3D_OPT_TILE_STAGE 2 with a cut): 1 runs
                                 each sweep as one launch, a workgroup per
                                 block marching the block's tile diagonals;
-                                0 one launch per diagonal over all blocks;
-                                -1 (default) 1.  Same values bit for bit. */
+                                2 so only the small diagonals at either end
+                                (fewer tiles than the device holds at
+                                once), one launch per other diagonal; 0 one
+                                launch per diagonal over all blocks; -1
+                                (default) 1 below 2^26 unknowns, else 0.
+                                Same values bit for bit. */
 };
 
 /* Creates a solver for one rank's grid (nr,nt,np include the boundary /

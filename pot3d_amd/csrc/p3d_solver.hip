@@ -936,12 +936,15 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   return P3D_OK;
 }
 
-// Block-persistent register-tile sweeps (P3D_OPT_BLOCK_SWEEP; -1: below
-// 2^26 unknowns, where one launch per diagonal leaves the device idle at
-// the launch boundaries; above, the blocks' uneven share of the CUs costs
-// more than the launches).
+// Block-persistent register-tile sweeps (P3D_OPT_BLOCK_SWEEP; -1: all
+// diagonals below 2^26 unknowns, where one launch per diagonal leaves the
+// device idle at the launch boundaries; above, one launch per diagonal:
+// there the blocks' uneven share of the CUs costs more than the launches,
+// and the forward sweep that also updates the residual runs slower
+// block-persistent, also for the small diagonals alone).
 static int block_sweep_on(const p3d_solver* s) {
-  return s->blk_sweep == 1 || (s->blk_sweep < 0 && s->g.N < (1u << 26));
+  if (s->blk_sweep >= 0) return s->blk_sweep;
+  return s->g.N < (1u << 26) ? 1 : 0;
 }
 
 // Block extents of the block-Jacobi cut for nblk blocks per axis, rounded
@@ -1059,7 +1062,7 @@ int p3d_set_option(p3d_solver* s, int key, double value) {
     }
     case P3D_OPT_BLOCK_SWEEP: {
       const int v = (int)value;
-      if (v < -1 || v > 1) return fail(P3D_EINVAL, "P3D_OPT_BLOCK_SWEEP must be -1, 0 or 1");
+      if (v < -1 || v > 2) return fail(P3D_EINVAL, "P3D_OPT_BLOCK_SWEEP must be -1, 0, 1 or 2");
       if (v != s->blk_sweep) {
         // captured iterations hold the other launch sequence
         HIPCHK(hipStreamSynchronize(s->stream));

@@ -1167,6 +1167,7 @@ struct BlkArgs {
   const int* tiles;  // tile codes, by block, then diagonal
   const int* off;    // nblk * ndiag + 1 offsets into tiles
   int ndiag;
+  int L0, L1;  // the diagonals this launch marches: L0 .. L1-1
 };
 
 template <typename T, int TK, bool kWrap, bool kUpd = false>
@@ -1185,7 +1186,7 @@ __global__ void __launch_bounds__(64 * kBlkWaves)
   float alpha = 0.f;
   if constexpr (kUpd) alpha = (float)st_load(&guard->alpha);
   const int* off = ba.off + (size_t)blockIdx.x * ba.ndiag;
-  for (int L = 0; L < ba.ndiag; L++) {
+  for (int L = ba.L0; L < ba.L1; L++) {
     const int o1 = off[L + 1];
     for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
       const int code = ba.tiles[b];
@@ -1213,7 +1214,7 @@ __global__ void __launch_bounds__(64 * kBlkWaves)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (guard && st_done(guard)) return;  // as in the forward sweep
   const int* off = ba.off + (size_t)blockIdx.x * ba.ndiag;
-  for (int L = ba.ndiag - 1; L >= 0; L--) {
+  for (int L = ba.L1 - 1; L >= ba.L0; L--) {
     const int o1 = off[L + 1];
     for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
       const int code = ba.tiles[b];
@@ -1246,74 +1247,100 @@ __global__ void k_reg_coup(Metric m, Grid3 g, int TNI, int TNJ, float* __restric
   u[128] = cu.i;
 }
 
+// One sweep's launches of the register tiles.  Diagonals [0, plan.ndiag)
+// in order (forward) or reverse (backward); with persist 1 one
+// block-persistent launch marches all of them, with persist 2 one marches
+// the leading diagonals [0, blk_lo) and one the trailing [blk_hi, ndiag)
+// (the small ones, latency-bound as launches of their own), the others one
+// launch each.
+template <typename T, int TK, bool kFwd>
+void tiles_sweep_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, T* r, T* z,
+                     const CgState* guard, hipStream_t s, const float* ap) {
+  const bool wrap = plan.bl.wrap != 0;
+  constexpr int kThreads = 64 * kRegTilesPerGroup;
+  const bool pers = plan.persist != 0 && plan.blk_tiles;
+  const int lo = !pers ? 0 : (plan.persist == 1 ? plan.ndiag : plan.blk_lo);
+  const int hi = !pers ? plan.ndiag : (plan.persist == 1 ? plan.ndiag : plan.blk_hi);
+  auto blk = [&](int L0, int L1) {
+    if (L0 >= L1) return;
+    DiagArgs da = diag_args(plan, 0);
+    da.tiles = nullptr;
+    da.count = 0;
+    const BlkArgs ba{plan.blk_tiles, plan.blk_off, plan.ndiag, L0, L1};
+    const int nb = plan.nblk;
+    constexpr int kT = 64 * kBlkWaves;
+    if constexpr (kFwd) {
+      bool upd = false;
+      if constexpr (sizeof(T) == 4) {
+        if (ap) {
+          upd = true;
+          if (wrap)
+            k_tile_fwd_blk<T, TK, true, true>
+                <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard, ap);
+          else
+            k_tile_fwd_blk<T, TK, false, true>
+                <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard, ap);
+        }
+      }
+      if (!upd) {
+        if (wrap)
+          k_tile_fwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
+        else
+          k_tile_fwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
+      }
+    } else {
+      if (wrap)
+        k_tile_bwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+      else
+        k_tile_bwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+    }
+  };
+  auto diag = [&](int L) {
+    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
+    const DiagArgs da = diag_args(plan, L);
+    const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
+    if constexpr (kFwd) {
+      if constexpr (sizeof(T) == 4) {
+        if (ap) {
+          if (wrap)
+            k_tile_fwd_reg<T, TK, true, true>
+                <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard, ap);
+          else
+            k_tile_fwd_reg<T, TK, false, true>
+                <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard, ap);
+          return;
+        }
+      }
+      if (wrap)
+        k_tile_fwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+      else
+        k_tile_fwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, r, guard);
+    } else {
+      if (wrap)
+        k_tile_bwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
+      else
+        k_tile_bwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
+    }
+  };
+  if constexpr (kFwd) {
+    blk(0, lo);
+    for (int L = lo; L < hi; L++) diag(L);
+    blk(max(hi, lo), plan.ndiag);
+  } else {
+    blk(max(hi, lo), plan.ndiag);
+    for (int L = hi - 1; L >= lo; L--) diag(L);
+    blk(0, lo);
+  }
+}
+
 // ap non-null (FP32, with guard): the forward sweep also applies the
 // residual update r -= alpha Ap (k_tile_fwd_reg<.., kUpd>).
 template <typename T, int TK>
 void tiles_launch_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, const T* r, T* z,
                       const CgState* guard, hipStream_t s, const float* ap = nullptr) {
-  const bool wrap = plan.bl.wrap != 0;
-  constexpr int kThreads = 64 * kRegTilesPerGroup;
   T* rw = const_cast<T*>(r);  // written only by the kUpd sweep
-  if (plan.persist && plan.blk_tiles) {
-    DiagArgs da = diag_args(plan, 0);
-    da.tiles = nullptr;
-    da.count = 0;
-    const BlkArgs ba{plan.blk_tiles, plan.blk_off, plan.ndiag};
-    const int nb = plan.nblk;
-    constexpr int kT = 64 * kBlkWaves;
-    bool upd = false;
-    if constexpr (sizeof(T) == 4) {
-      if (ap) {
-        upd = true;
-        if (wrap)
-          k_tile_fwd_blk<T, TK, true, true>
-              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard, ap);
-        else
-          k_tile_fwd_blk<T, TK, false, true>
-              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard, ap);
-      }
-    }
-    if (!upd) {
-      if (wrap)
-        k_tile_fwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard);
-      else
-        k_tile_fwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, rw, guard);
-    }
-    if (wrap)
-      k_tile_bwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
-    else
-      k_tile_bwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
-    return;
-  }
-  for (int L = 0; L < plan.ndiag; L++) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    const DiagArgs da = diag_args(plan, L);
-    const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
-    if constexpr (sizeof(T) == 4) {
-      if (ap) {
-        if (wrap)
-          k_tile_fwd_reg<T, TK, true, true>
-              <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard, ap);
-        else
-          k_tile_fwd_reg<T, TK, false, true>
-              <<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard, ap);
-        continue;
-      }
-    }
-    if (wrap)
-      k_tile_fwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard);
-    else
-      k_tile_fwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_lo, rw, guard);
-  }
-  for (int L = plan.ndiag - 1; L >= 0; L--) {
-    const int cnt = plan.diag_off[L + 1] - plan.diag_off[L];
-    const DiagArgs da = diag_args(plan, L);
-    const int nb = (cnt + kRegTilesPerGroup - 1) / kRegTilesPerGroup;
-    if (wrap)
-      k_tile_bwd_reg<T, TK, true><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
-    else
-      k_tile_bwd_reg<T, TK, false><<<nb, kThreads, 0, s>>>(m, g, da, plan.coup_up, z, guard);
-  }
+  tiles_sweep_reg<T, TK, true>(plan, m, g, rw, z, guard, s, ap);
+  tiles_sweep_reg<T, TK, false>(plan, m, g, rw, z, guard, s, ap);
 }
 
 template <typename T, int TK, int W>
@@ -1754,6 +1781,14 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     }
     hipMemcpy(plan.blk_tiles, bt.data(), bt.size() * sizeof(int), hipMemcpyHostToDevice);
     hipMemcpy(plan.blk_off, bo.data(), bo.size() * sizeof(int), hipMemcpyHostToDevice);
+    // the small diagonals at either end: fewer tiles than the device holds
+    // at once (24 waves per CU)
+    const int small = 24 * (num_cus > 0 ? num_cus : 256);
+    auto cnt = [&](int L) { return plan.diag_off[L + 1] - plan.diag_off[L]; };
+    plan.blk_lo = 0;
+    while (plan.blk_lo < plan.ndiag && cnt(plan.blk_lo) < small) plan.blk_lo++;
+    plan.blk_hi = plan.ndiag;
+    while (plan.blk_hi > plan.blk_lo && cnt(plan.blk_hi - 1) < small) plan.blk_hi--;
   }
   if (plan.SI == 1 && plan.SJ == 1) {
     // tile-major dinv and y of the single-wave sweeps: every tile reads and

@@ -52,10 +52,13 @@ struct SweepPlan {
   // block-Jacobi register tiles: the tiles of each of the nblk blocks by
   // diagonal (blk_off: nblk * ndiag + 1 offsets into blk_tiles); persist 1
   // runs each sweep as one launch, a workgroup per block marching its
-  // diagonals, 0 as one launch per diagonal (the same bits)
+  // diagonals, 2 so only the small diagonals [0, blk_lo) and [blk_hi,
+  // ndiag) at either end (one launch each) and the others one launch per
+  // diagonal, 0 one launch per diagonal (all the same bits)
   int nblk = 0;
   int* blk_tiles = nullptr;
   int* blk_off = nullptr;
+  int blk_lo = 0, blk_hi = 0;
   int persist = 0;
 };
 
