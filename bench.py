@@ -262,7 +262,7 @@ def main():
                     help="1 residual rule, 2 A-norm rule for accepting the learned guess")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path "
               f"(--gpus {args.gpus} requested); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
