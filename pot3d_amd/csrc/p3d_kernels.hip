@@ -1054,7 +1054,8 @@ __global__ void __launch_bounds__(kBlock)
 
 void launch_r_update(uint32_t n_all, const Launch& Lt, float* r, const float* ap,
                      const CgState* st, hipStream_t s) {
-  k_r_update<<<Lt.G, kBlock, 0, s>>>(n_all, Lt.G, r, ap, st);
+  const int G = Lt.Gru > 0 ? Lt.Gru : Lt.G;
+  k_r_update<<<G, kBlock, 0, s>>>(n_all, G, r, ap, st);
 }
 
 // p = beta p + z  (:4361-4363)
@@ -1224,20 +1225,36 @@ __global__ void __launch_bounds__(kBlock)
 void launch_p_update_x(const Grid3& g, const Launch& L, int t, const PBuf& pb, const float* z,
                        CgState* st, hipStream_t s, const TileMap& tm, double* xs,
                        const double* xscale, unsigned* counter) {
+  const int G = L.Gpx[t & 3] > 0 ? L.Gpx[t & 3] : L.G;
   switch (t) {
     case 0:
-      k_p_update_x<0><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      k_p_update_x<0><<<G, kBlock, 0, s>>>(g, G, L.CH, pb, z, st, tm, xs, xscale, counter);
       break;
     case 1:
-      k_p_update_x<1><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      k_p_update_x<1><<<G, kBlock, 0, s>>>(g, G, L.CH, pb, z, st, tm, xs, xscale, counter);
       break;
     case 2:
-      k_p_update_x<2><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      k_p_update_x<2><<<G, kBlock, 0, s>>>(g, G, L.CH, pb, z, st, tm, xs, xscale, counter);
       break;
     default:
-      k_p_update_x<3><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, pb, z, st, tm, xs, xscale, counter);
+      k_p_update_x<3><<<G, kBlock, 0, s>>>(g, G, L.CH, pb, z, st, tm, xs, xscale, counter);
       break;
   }
+}
+
+void size_stream_grids(Launch& L, int ncu) {
+  auto fit = [&](auto kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess ||
+        nb < 1 || nb * ncu >= L.G)
+      return 0;
+    return nb * ncu;
+  };
+  L.Gpx[0] = fit(k_p_update_x<0>);
+  L.Gpx[1] = fit(k_p_update_x<1>);
+  L.Gpx[2] = fit(k_p_update_x<2>);
+  L.Gpx[3] = fit(k_p_update_x<3>);
+  L.Gru = fit(k_r_update);
 }
 
 template <typename T>

@@ -92,6 +92,12 @@ struct Launch {
   // operator kernel: 0 column march, 1 row-blocked march, 2 LDS-staged
   // patches (3 tall, 4 short), -1 the LDS-staged patches where they apply
   int stencil = -1;
+  // grid-stride kernels whose registers allow fewer than G resident
+  // workgroups (the grouped p update, t = 0..3, and the FP32 r update):
+  // as many workgroups as the device holds at once (0: G), so that no
+  // second wave of workgroups streams its share alone at the end
+  int Gpx[4] = {0, 0, 0, 0};
+  int Gru = 0;
 };
 
 constexpr int kBlock = 256;
@@ -104,6 +110,8 @@ constexpr int kUnroll = 4;  // points per thread in flight in the streaming kern
 constexpr int kPoleSegs = 16;
 
 Launch plan_stream(uint32_t N, int max_blocks);
+// Fills L.Gpx and L.Gru from the kernels' occupancy on ncu CUs.
+void size_stream_grids(Launch& L, int ncu);
 
 // Pole averages (m=0 component) of x at j=0 and j=b-1, per interior i, as
 // kPoleSegs partial sums each: part[(pole*kPoleSegs+seg)*a + i].
