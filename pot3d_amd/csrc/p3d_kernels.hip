@@ -1013,8 +1013,13 @@ template <typename T>
 void launch_cg_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* x, T* r,
                       const T* p, const T* ap, CgState* st, Reduce red, int slot,
                       hipStream_t s, double* xs, const double* xscale, const TileMap& tm) {
-  k_cg_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, x, r, p, ap, st, red.partial,
-                                         red.counter + (size_t)slot * kRedCounters, xs, xscale, tm);
+  int G = L.G, CH = L.CH;
+  if (sizeof(T) == 8 && L.Gcg > 0) {
+    G = L.Gcg;
+    CH = L.CHcg;
+  }
+  k_cg_update<T><<<G, kBlock, 0, s>>>(m, g, G, CH, prec, x, r, p, ap, st, red.partial,
+                                       red.counter + (size_t)slot * kRedCounters, xs, xscale, tm);
 }
 
 // The residual update of the FP32 recurrence with the FP64 solution, one
@@ -1242,27 +1247,13 @@ void launch_p_update_x(const Grid3& g, const Launch& L, int t, const PBuf& pb, c
   }
 }
 
-void size_stream_grids(Launch& L, int ncu) {
-  auto fit = [&](auto kern) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess ||
-        nb < 1 || nb * ncu >= L.G)
-      return 0;
-    return nb * ncu;
-  };
-  L.Gpx[0] = fit(k_p_update_x<0>);
-  L.Gpx[1] = fit(k_p_update_x<1>);
-  L.Gpx[2] = fit(k_p_update_x<2>);
-  L.Gpx[3] = fit(k_p_update_x<3>);
-  L.Gru = fit(k_r_update);
-}
 
 template <typename T>
 void launch_p_update(const Metric& m, const Grid3& g, const Launch& L, int prec, T* p,
                      const T* r, const T* z, CgState* st, hipStream_t s, const TileMap& tm,
                      double* xs, const double* xscale, unsigned* counter) {
-  k_p_update<T><<<L.G, kBlock, 0, s>>>(m, g, L.G, L.CH, prec, p, r, z, st, tm, xs, xscale,
-                                        counter);
+  const int G = sizeof(T) == 8 && L.Gp > 0 ? L.Gp : L.G;
+  k_p_update<T><<<G, kBlock, 0, s>>>(m, g, G, L.CH, prec, p, r, z, st, tm, xs, xscale, counter);
 }
 
 // s0 = x.y, s1 = u.v
@@ -1313,8 +1304,10 @@ template <typename T>
 void launch_dot2(const Grid3& g, const Launch& L, const T* x, const T* y, const T* u,
                  const T* v, CgState* st, Reduce red, int slot, Epi epi, double* out,
                  bool guard, hipStream_t s, const TileMap& tm) {
-  k_dot2<T><<<L.G, kBlock, 0, s>>>(g, L.G, L.CH, x, y, u, v, st, red.partial,
-                                    red.counter + (size_t)slot * kRedCounters, epi, out, guard ? 1 : 0, tm);
+  const int G = sizeof(T) == 8 && L.Gd2 > 0 ? L.Gd2 : L.G;
+  k_dot2<T><<<G, kBlock, 0, s>>>(g, G, L.CH, x, y, u, v, st, red.partial,
+                                  red.counter + (size_t)slot * kRedCounters, epi, out,
+                                  guard ? 1 : 0, tm);
 }
 
 // z = M^-1 r (DIAG/NONE), reductions s0=r.z, s1=r.r.
@@ -1873,5 +1866,29 @@ template void launch_cast_scale<double, float>(const Grid3&, double*, const floa
 template void launch_cast_scale<double, double>(const Grid3&, double*, const double*,
                                                 const double*, const double*, int, hipStream_t,
                                                 const TileMap&);
+
+void size_stream_grids(Launch& L, int ncu, uint32_t N) {
+  // a multiple of 8 (the blocks' XCD mapping), below L.G, or 0 (L.G)
+  auto fit = [&](auto kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess ||
+        nb < 1)
+      return 0;
+    const int G = nb * ncu / 8 * 8;
+    return G >= 8 && G < L.G ? G : 0;
+  };
+  L.Gcg = fit(k_cg_update<double>);
+  if (L.Gcg > 0) {
+    const uint32_t ch = (N + (uint32_t)L.Gcg - 1) / (uint32_t)L.Gcg;
+    L.CHcg = (int)((ch + kBlock - 1) / kBlock * kBlock);
+  }
+  L.Gp = fit(k_p_update<double>);
+  L.Gd2 = fit(k_dot2<double>);
+  L.Gpx[0] = fit(k_p_update_x<0>);
+  L.Gpx[1] = fit(k_p_update_x<1>);
+  L.Gpx[2] = fit(k_p_update_x<2>);
+  L.Gpx[3] = fit(k_p_update_x<3>);
+  L.Gru = fit(k_r_update);
+}
 
 }  // namespace p3d

@@ -98,6 +98,9 @@ struct Launch {
   // second wave of workgroups streams its share alone at the end
   int Gpx[4] = {0, 0, 0, 0};
   int Gru = 0;
+  // the same for the FP64 CG update (chunks of CHcg points), p update and
+  // dot products
+  int Gcg = 0, CHcg = 0, Gp = 0, Gd2 = 0;
 };
 
 constexpr int kBlock = 256;
@@ -110,8 +113,9 @@ constexpr int kUnroll = 4;  // points per thread in flight in the streaming kern
 constexpr int kPoleSegs = 16;
 
 Launch plan_stream(uint32_t N, int max_blocks);
-// Fills L.Gpx and L.Gru from the kernels' occupancy on ncu CUs.
-void size_stream_grids(Launch& L, int ncu);
+// Fills L.Gpx, L.Gru, L.Gcg/CHcg, L.Gp and L.Gd2 from the kernels'
+// occupancy on ncu CUs, for passes over N points.
+void size_stream_grids(Launch& L, int ncu, uint32_t N);
 
 // Pole averages (m=0 component) of x at j=0 and j=b-1, per interior i, as
 // kPoleSegs partial sums each: part[(pole*kPoleSegs+seg)*a + i].

@@ -192,7 +192,7 @@ static int update_tilemap(p3d_solver* s) {
     s->gt.N = (uint32_t)s->sweep.tile_elems;
     // as many blocks as the reduction buffers hold (a multiple of 8)
     s->Lt = plan_stream(s->gt.N, ncu * 8 < s->red.max_blocks ? ncu * 8 : s->red.max_blocks);
-    size_stream_grids(s->Lt, ncu);
+    size_stream_grids(s->Lt, ncu, s->gt.N);
   }
   HIPCHK(hipMemsetAsync(s->r, 0, s->nvec * sizeof(double), s->stream));
   HIPCHK(hipMemsetAsync(s->z, 0, s->nvec * sizeof(double), s->stream));
@@ -775,7 +775,7 @@ int p3d_create(p3d_solver** out, int nr, int nt, int np, const double* r, const 
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     ncu = prop.multiProcessorCount;
   s->L = plan_stream((uint32_t)N, ncu * 8);
-  size_stream_grids(s->L, ncu);
+  size_stream_grids(s->L, ncu, (uint32_t)N);
   int rc = P3D_OK;
   auto bail = [&](int code) {
     p3d_destroy(s);
