@@ -802,39 +802,64 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(C::
   block_finish<1>(v, partial, counter, G, Lb, st, epi, out);
 }
 
-template <typename C>
+// Workgroups of k_stencil_lds<T, C> the device holds at once (its
+// occupancy on every CU; the same on every device of the process).
+template <typename T, typename C>
+static int stencil_lds_resident() {
+  static const int R = [] {
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+      ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stencil_lds<T, C>, kBlock, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 2;
+    return nb * ncu;
+  }();
+  return R;
+}
+
+// Segments per patch column: whole rounds of the workgroups the device
+// holds at once (about 4096 workgroups; a last round partly filled would
+// run nearly alone), segments of >= kSegPlanes and <= kLdsMaxPlanes planes.
+template <typename T, typename C>
 static int stencil_lds_segments(const Grid3& g) {
-  // about 4 rounds of 4 workgroups per CU, >= kSegPlanes planes each
   int nwi, nwj;
   stencil_lds_shape<C>(g, nwi, nwj);
-  int nseg = (4 * 4 * 256 + nwi * nwj - 1) / (nwi * nwj);
+  const int P = nwi * nwj, R = stencil_lds_resident<T, C>();
+  const int rounds = std::max(1, (4096 + R / 2) / R);
+  int nseg = rounds * R / P;
   const int maxseg = (g.c + kSegPlanes - 1) / kSegPlanes;
-  if (nseg > maxseg) nseg = maxseg;
-  // segments of at most kLdsMaxPlanes planes
+  if (nseg > maxseg) {
+    const int fit = maxseg * P / R;  // whole rounds that fit
+    nseg = fit >= 1 ? fit * R / P : maxseg;
+  }
   nseg = std::max(nseg, (g.c + kLdsMaxPlanes - 1) / kLdsMaxPlanes);
   if (nseg < 1) nseg = 1;
   return nseg;
 }
 
-template <typename C>
+template <typename T, typename C>
 static int stencil_lds_blocks(const Grid3& g) {
   int nwi, nwj;
   stencil_lds_shape<C>(g, nwi, nwj);
-  const int G = nwi * nwj * stencil_lds_segments<C>(g);
+  const int G = nwi * nwj * stencil_lds_segments<T, C>(g);
   return (G + 7) / 8 * 8;
 }
 
 static int stencil_lds_blocks_max(const Grid3& g) {
-  return std::max({stencil_lds_blocks<LdsCfg<double, true>>(g),
-                   stencil_lds_blocks<LdsCfg<double, false>>(g),
-                   stencil_lds_blocks<LdsCfg<float, true>>(g),
-                   stencil_lds_blocks<LdsCfg<float, false>>(g)});
+  return std::max({stencil_lds_blocks<double, LdsCfg<double, true>>(g),
+                   stencil_lds_blocks<double, LdsCfg<double, false>>(g),
+                   stencil_lds_blocks<float, LdsCfg<float, true>>(g),
+                   stencil_lds_blocks<float, LdsCfg<float, false>>(g)});
 }
 
 // The tall patches when they still make about 8 workgroups per CU.
 template <typename T>
 static bool stencil_lds_tall(const Grid3& g) {
-  return stencil_lds_blocks<LdsCfg<T, true>>(g) >= 8 * 256;
+  return stencil_lds_blocks<T, LdsCfg<T, true>>(g) >= 8 * 256;
 }
 
 template <typename T, typename C>
@@ -842,8 +867,8 @@ static void launch_stencil_lds(const Metric& m, const Grid3& g, const T* x, T* y
                                const double* pole_part, CgState* st, Reduce red, int slot,
                                Epi epi, double* out, bool guard, int want, hipStream_t s,
                                const TileMap& tm) {
-  const int nseg = stencil_lds_segments<C>(g);
-  const int G = stencil_lds_blocks<C>(g);
+  const int nseg = stencil_lds_segments<T, C>(g);
+  const int G = stencil_lds_blocks<T, C>(g);
   k_stencil_lds<T, C><<<G, kBlock, 0, s>>>(m, g, nseg, x, y, pole_part, st, red.partial,
                                             red.counter + (size_t)slot * kRedCounters, epi, out,
                                             guard ? 1 : 0, want, tm);
