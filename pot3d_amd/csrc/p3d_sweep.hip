@@ -1157,11 +1157,13 @@ __global__ void __launch_bounds__(64 * kRegTilesPerGroup)
 // them for the workgroup); the per-tile work is that of k_tile_fwd_reg /
 // k_tile_bwd_reg, so the results are the same bits.
 constexpr int kBlkWaves = 8;
-// Waves per SIMD of the block-persistent sweeps: three workgroups per CU
-// for the FP32 16-plane tiles without the seam (the loop over diagonals
-// keeps more values live than one tile does).
-template <typename T, int TK, bool kWrap>
-constexpr int kBlkRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? 6 : 1;
+// Waves per SIMD of the block-persistent sweeps for the FP32 16-plane tiles
+// without the seam: with kRoomy (at most 2 blocks per CU) 4, i.e. two
+// workgroups per CU and registers enough for the loop over diagonals
+// without spilling; else 6, three workgroups per CU (the loop then spills
+// a few registers).
+template <typename T, int TK, bool kWrap, bool kRoomy>
+constexpr int kBlkRegWaves = (TK == 16 && !kWrap && sizeof(T) == 4) ? (kRoomy ? 4 : 6) : 1;
 
 struct BlkArgs {
   const int* tiles;  // tile codes, by block, then diagonal
@@ -1170,9 +1172,9 @@ struct BlkArgs {
   int L0, L1;  // the diagonals this launch marches: L0 .. L1-1
 };
 
-template <typename T, int TK, bool kWrap, bool kUpd = false>
+template <typename T, int TK, bool kWrap, bool kUpd, bool kRoomy>
 __global__ void __launch_bounds__(64 * kBlkWaves)
-    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap>, 8)))
+    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap, kRoomy>, 8)))
     k_tile_fwd_blk(Metric m, Grid3 g, DiagArgs da, BlkArgs ba, const float* __restrict__ coup,
                    T* __restrict__ r, const CgState* guard,
                    const float* __restrict__ ap = nullptr) {
@@ -1203,9 +1205,9 @@ __global__ void __launch_bounds__(64 * kBlkWaves)
   }
 }
 
-template <typename T, int TK, bool kWrap>
+template <typename T, int TK, bool kWrap, bool kRoomy>
 __global__ void __launch_bounds__(64 * kBlkWaves)
-    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap>, 8)))
+    __attribute__((amdgpu_waves_per_eu(kBlkRegWaves<T, TK, kWrap, kRoomy>, 8)))
     k_tile_bwd_blk(Metric m, Grid3 g, DiagArgs da, BlkArgs ba, const float* __restrict__ coup,
                    T* __restrict__ z, const CgState* guard) {
   static_assert(TK >= SKEW && TK + 2 * SKEW <= 64, "register tiles: 16 or 32 planes");
@@ -1269,30 +1271,46 @@ void tiles_sweep_reg(const SweepPlan& plan, const Metric& m, const Grid3& g, T* 
     const BlkArgs ba{plan.blk_tiles, plan.blk_off, plan.ndiag, L0, L1};
     const int nb = plan.nblk;
     constexpr int kT = 64 * kBlkWaves;
+    // the register budget only differs for the FP32 16-plane tiles
+    // without the seam
+    constexpr bool kTuned = TK == 16 && sizeof(T) == 4;
+    const bool roomy = kTuned && plan.blk_roomy;
     if constexpr (kFwd) {
       bool upd = false;
       if constexpr (sizeof(T) == 4) {
         if (ap) {
           upd = true;
           if (wrap)
-            k_tile_fwd_blk<T, TK, true, true>
+            k_tile_fwd_blk<T, TK, true, true, false>
+                <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard, ap);
+          else if (roomy)
+            k_tile_fwd_blk<T, TK, false, true, kTuned>
                 <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard, ap);
           else
-            k_tile_fwd_blk<T, TK, false, true>
+            k_tile_fwd_blk<T, TK, false, true, false>
                 <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard, ap);
         }
       }
       if (!upd) {
         if (wrap)
-          k_tile_fwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
+          k_tile_fwd_blk<T, TK, true, false, false>
+              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
+        else if (roomy)
+          k_tile_fwd_blk<T, TK, false, false, kTuned>
+              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
         else
-          k_tile_fwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
+          k_tile_fwd_blk<T, TK, false, false, false>
+              <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_lo, r, guard);
       }
     } else {
       if (wrap)
-        k_tile_bwd_blk<T, TK, true><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+        k_tile_bwd_blk<T, TK, true, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+      else if (roomy)
+        k_tile_bwd_blk<T, TK, false, kTuned>
+            <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
       else
-        k_tile_bwd_blk<T, TK, false><<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
+        k_tile_bwd_blk<T, TK, false, false>
+            <<<nb, kT, 0, s>>>(m, g, da, ba, plan.coup_up, z, guard);
     }
   };
   auto diag = [&](int L) {
@@ -1789,6 +1807,8 @@ int build_sweep_plan(SweepPlan& plan, const Grid3& g, int num_cus, int tile_k, i
     while (plan.blk_lo < plan.ndiag && cnt(plan.blk_lo) < small) plan.blk_lo++;
     plan.blk_hi = plan.ndiag;
     while (plan.blk_hi > plan.blk_lo && cnt(plan.blk_hi - 1) < small) plan.blk_hi--;
+    // two 8-wave workgroups per CU hold every block at once
+    plan.blk_roomy = plan.nblk <= 2 * (num_cus > 0 ? num_cus : 256);
   }
   if (plan.SI == 1 && plan.SJ == 1) {
     // tile-major dinv and y of the single-wave sweeps: every tile reads and

@@ -59,6 +59,7 @@ struct SweepPlan {
   int* blk_tiles = nullptr;
   int* blk_off = nullptr;
   int blk_lo = 0, blk_hi = 0;
+  int blk_roomy = 0;  // nblk <= 2 per CU: the block kernels' larger register budget
   int persist = 0;
 };
 
