@@ -847,8 +847,12 @@ __device__ __forceinline__ void reg_stage(TileRegLds<TK>& sh, const RegIn<TK>& i
 }
 
 // kUpd: the residual update of the FP32 recurrence on the way in,
-// r = r - alpha Ap (the tile's r is stored back by the caller).
-template <typename T, int TK, bool kWrap, bool kUpd = false>
+// r = r - alpha Ap (the tile's r is stored back by the caller).  kCoup:
+// the column's couplings from the per-column table (the block-persistent
+// sweeps, whose registers are the tighter), else formed from the metric's
+// 1-D factors as the table was (the same values; the per-diagonal
+// launches, where the table's 2.2 MB are not kept in the L2s).
+template <typename T, int TK, bool kWrap, bool kUpd = false, bool kCoup = false>
 __device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, const Grid3& g,
                                              const DiagArgs& da, const float* coup,
                                              const T* __restrict__ r, int code, int lane,
@@ -867,9 +871,18 @@ __device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, con
   in.pf = make_float2(m.FDPH[kq], m.FDPL[kq]);
   in.fk = da.face_k[tile_id(da, t.I, t.J, max(t.Kc - 1, 0)) * 64 + lane];
   in.fkt = kWrap ? da.face_kt[t.col * 64 + lane] : 0.f;
-  in.ck = coup[(t.col * 3 + 0) * 64 + lane];
-  in.cj = coup[(t.col * 3 + 1) * 64 + lane];
-  in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+  if constexpr (kCoup) {
+    in.ck = coup[(t.col * 3 + 0) * 64 + lane];
+    in.cj = coup[(t.col * 3 + 1) * 64 + lane];
+    in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+  } else {
+    const int i = t.I * TI + (lane & 7), j = t.J * TJ + (lane >> 3);
+    const bool act = i < g.a && j < g.b;
+    const SweepCoup c = sweep_coup_lower(m, act ? i : 0, act ? j : 0);
+    in.ck = c.k;
+    in.cj = c.j;
+    in.ci = c.i;
+  }
 #pragma unroll
   for (int u = 0; u < TK; u++) {
     in.d[u] = da.dinv_t[t.tb + (size_t)u * 64];
@@ -880,7 +893,7 @@ __device__ __forceinline__ void fwd_reg_load(RegIn<TK>& in, const Metric& m, con
   }
 }
 
-template <typename T, int TK, bool kWrap>
+template <typename T, int TK, bool kWrap, bool kCoup = false>
 __device__ __forceinline__ void bwd_reg_load(RegIn<TK>& in, const Metric& m, const Grid3& g,
                                              const DiagArgs& da, const float* coup, int code,
                                              int lane) {
@@ -897,9 +910,18 @@ __device__ __forceinline__ void bwd_reg_load(RegIn<TK>& in, const Metric& m, con
   in.pf = make_float2(m.FDPH[kq], m.FDPU[kq]);
   in.fk = da.face_k[tile_id(da, t.I, t.J, min(t.Kc + 1, da.NK - 1)) * 64 + lane];
   in.fkt = kWrap ? da.face_kt[t.col * 64 + lane] : 0.f;
-  in.ck = coup[(t.col * 3 + 0) * 64 + lane];
-  in.cj = coup[(t.col * 3 + 1) * 64 + lane];
-  in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+  if constexpr (kCoup) {
+    in.ck = coup[(t.col * 3 + 0) * 64 + lane];
+    in.cj = coup[(t.col * 3 + 1) * 64 + lane];
+    in.ci = coup[(t.col * 3 + 2) * 64 + lane];
+  } else {
+    const int i = t.I * TI + (lane & 7), j = t.J * TJ + (lane >> 3);
+    const bool act = i < g.a && j < g.b;
+    const SweepCoup c = sweep_coup_upper(m, act ? i : 0, act ? j : 0);
+    in.ck = c.k;
+    in.cj = c.j;
+    in.ci = c.i;
+  }
 #pragma unroll
   for (int u = 0; u < TK; u++) {
     in.v[u] = da.yv_t[t.tb + (size_t)u * 64];
@@ -1193,7 +1215,7 @@ __global__ void __launch_bounds__(64 * kBlkWaves)
     for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
       const int code = ba.tiles[b];
       RegIn<TK> in;
-      fwd_reg_load<T, TK, kWrap, kUpd>(in, m, g, da, coup, r, code, lane, ap, alpha);
+      fwd_reg_load<T, TK, kWrap, kUpd, true>(in, m, g, da, coup, r, code, lane, ap, alpha);
       if constexpr (kUpd) {
         const RegTile<TK> t(da, g, code, lane);
 #pragma unroll
@@ -1221,7 +1243,7 @@ __global__ void __launch_bounds__(64 * kBlkWaves)
     for (int b = off[L] + wave; b < o1; b += kBlkWaves) {
       const int code = ba.tiles[b];
       RegIn<TK> in;
-      bwd_reg_load<T, TK, kWrap>(in, m, g, da, coup, code, lane);
+      bwd_reg_load<T, TK, kWrap, true>(in, m, g, da, coup, code, lane);
       bwd_reg_tile<T, TK, kWrap>(in, shw[wave], m, g, da, z, code, lane);
     }
     __syncthreads();
